@@ -1,0 +1,161 @@
+/*
+ * libpgj: MI355X (gfx950) kernels for the partial Green-Jacobi homogenization
+ * hot path, behind a plain C ABI.
+ *
+ * Conventions shared by every entry point:
+ *  - all array arguments are device pointers (HBM) unless marked "host";
+ *  - grids are (nz, ny, nx); node = (z*ny + y)*nx + x; displacement-like
+ *    vectors are AoS, (L, 3N) load-major, node-major, 3 components per node;
+ *  - `stream` is a hipStream_t passed as void*; every call only enqueues work
+ *    on that stream: no allocation, no synchronization, no memset, so every
+ *    call is safe inside hip stream capture (hipGraph);
+ *  - buffers named `ws`, `plan`, `state` are caller-owned device memory sized
+ *    by the matching *_bytes() query; `state` arrays are documented below;
+ *  - the return value is 0 on success, a PGJ_ERR_* code otherwise, and
+ *    pgj_last_error() then returns a thread-local message.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference checkout).
+ */
+#ifndef PGJ_H
+#define PGJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PGJ_OK = 0, PGJ_ERR_ARG = 1, PGJ_ERR_HIP = 2, PGJ_ERR_UNSUPPORTED = 3 };
+
+/* Geometry and reference medium of a periodic grid (host struct). The
+ * element material is a(x) * C0(lam, mu); the Green reference medium is C0. */
+typedef struct pgj_grid {
+  int32_t nz, ny, nx;
+  int32_t reserved;
+  double hx, hy, hz;
+  double lam, mu;
+} pgj_grid;
+
+int pgj_version(void);
+const char* pgj_last_error(void);
+
+/* ---------------------------------------------------------------- operator */
+
+/* out_l = K(a) u_l for each of L loads; loads with active[l] == 0 get zeros.
+ * active may be NULL (all active). Replaces the scalar-a kernel `apply`
+ * (exp/asg_sctop_reproduction_20261105/scalar_fi.cu:46-53,
+ *  exp/asg_gpu_resident_20261103/scalar_fi.cu:46-53) and, for C = a*C0, the
+ * general `apply` (solver/src/materials/hybrid_kernels.cu:117-127). */
+int pgj_hex8_apply(const pgj_grid* g, int L, const double* a, const double* u, const uint8_t* active,
+                   double* out, void* stream);
+
+/* Same operator by an independent thread-per-node kernel; used as the
+ * certification operator (the reference certifies with its general kernel,
+ * exp/asg_gpu_resident_20261103/resident.py:100-102). */
+int pgj_hex8_apply_reference(const pgj_grid* g, int L, const double* a, const double* u, double* out,
+                             void* stream);
+
+/* f = -sum_e a_e Ke chi(macro), shape (1, 3N); macro is 6 host doubles in
+ * Voigt order with engineering shear. Replaces `rhs`
+ * (solver/src/materials/hybrid_kernels.cu:128-137). */
+int pgj_hex8_rhs(const pgj_grid* g, const double* a, const double macro[6], double* out, void* stream);
+
+/* Element energies e_l,e = (u_e + chi_l)^T Ke(C0) (u_e + chi_l) / V, out (L, N);
+ * macros is L*6 host doubles. Replaces `evaluate`
+ * (exp/asg_scenarios_20261105/energy.cu:26-35). */
+int pgj_hex8_energy(const pgj_grid* g, int L, const double* u, const double* macros, double* out,
+                    void* stream);
+
+/* Homogenized tensor C^H (36 doubles, row-major) from the six unit-strain
+ * solutions u6 (6, 3N): C_ij = < a (u_i + chi_i)^T Ke (u_j + chi_j) > / V.
+ * Replaces PaperResident.effective_tensor
+ * (exp/asg_sctop_reproduction_20261105/reproduction.py:166-176). */
+size_t pgj_effective_tensor_ws_bytes(const pgj_grid* g);
+int pgj_effective_tensor(const pgj_grid* g, const double* a, const double* u6, double* ws, double* out36,
+                         void* stream);
+
+/* Mean stress <a C0 (eps(macro) + B_bar u)> (6 doubles). Replaces `stress`
+ * (solver/src/materials/hybrid_kernels.cu:138-147). */
+size_t pgj_avg_stress_ws_bytes(const pgj_grid* g);
+int pgj_hex8_avg_stress(const pgj_grid* g, const double* a, const double* u, const double macro[6], double* ws,
+                        double* out6, void* stream);
+
+/* Nodal diagonal d of K(a) and the partial Green-Jacobi scale
+ * s = exp(-alpha/2 (log d - mean log d)) over all 3N entries, both (1, 3N).
+ * d may be NULL. Replaces `diagonal`/`set_alpha`
+ * (solver/experiments/adaptive_green_scaling/study.py:80-101) and the inline
+ * form in Resident.select (exp/asg_gpu_resident_20261103/resident.py:170-174). */
+size_t pgj_scale_ws_bytes(const pgj_grid* g);
+int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alpha, double* d, double* s, double* ws,
+                       void* stream);
+
+/* Zero-mean projection per load and component, in place.
+ * Replaces `project_zero_mean` (solver/src/materials/hybrid_adapter.py:78-80). */
+size_t pgj_project_ws_bytes(const pgj_grid* g, int L);
+int pgj_project(const pgj_grid* g, int L, double* v, double* ws, void* stream);
+
+/* ------------------------------------------------------------ Green / FFT */
+
+/* Plan: FFT twiddles and the dense symmetric Green symbol (6 coefficients per
+ * half-spectrum frequency, [G00,G11,G22,G01,G02,G12], 1/N folded in).
+ * Replaces `fi_coeff` + `prepare`
+ * (solver/src/reference_medium/formula_kernels.cu:199-203,
+ *  solver/src/spectral/precision_kernels.cu:129-142) and the cuFFT plans
+ * (solver/src/spectral/precision_fft.cu:34-56). Supported axis lengths:
+ * products of 2, 3, 5 (and 7), each <= 256. */
+size_t pgj_plan_bytes(const pgj_grid* g);
+int pgj_plan_init(const pgj_grid* g, void* plan, void* stream);
+/* Copies the symbol out of a plan as (6, nz, ny, nx/2+1) doubles. */
+int pgj_plan_symbol(const pgj_grid* g, const void* plan, double* out, void* stream);
+
+/* Spectrum of the scale s (3, nz, ny, nx/2+1) complex128, needed by the
+ * fused projection of M_alpha; recompute whenever s changes. */
+size_t pgj_precond_ws_bytes(const pgj_grid* g, int L);
+int pgj_scale_spectrum(const pgj_grid* g, const void* plan, const double* s, void* s_hat, void* ws, void* stream);
+
+/* z_l = P s G s P r_l (s NULL: z = P G P r), L loads. Replaces the native
+ * Green apply plus projection/scaling
+ * (solver/src/spectral/native_adapter.py:23-33,
+ *  exp/asg_gpu_resident_20261103/resident.py:104-117,
+ *  solver/experiments/adaptive_green_scaling/study.py:91-101). */
+int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, const double* r, const double* s,
+                      const void* s_hat, double* z, void* ws, void* stream);
+
+/* ------------------------------------------------------ resident PCG (FP64) */
+
+/* Per-load solver state, `state` of pgj_resident_state_bytes(L) bytes:
+ *   double rz[L], target2[L], norm[L], absolute[L];
+ *   int64  iterations[L];
+ *   uint8  active[L], bad[L]   (8-byte aligned blocks, in this order),
+ * followed by private fields. Offsets are given by pgj_resident_state_offsets. */
+size_t pgj_resident_state_bytes(int L);
+/* offsets[0..6] = byte offsets of rz, target2, norm, absolute, iterations, active, bad. */
+int pgj_resident_state_offsets(int L, size_t* offsets7);
+size_t pgj_resident_ws_bytes(const pgj_grid* g, int L);
+
+/* rhs_l for the L macro strains (host L*6), norms, targets
+ * max(rtol |f|, 1e-13/sqrt(nx))^2, iterations = 0, bad = 0.
+ * Replaces Resident._solve_batch set-up (resident.py:186-191). */
+int pgj_resident_setup(const pgj_grid* g, int L, const double* a, const double* macros, double rtol,
+                       double* rhs, void* state, void* ws, void* stream);
+/* u <- P u; r = rhs - K u (reference operator); absolute = |r|.
+ * Replaces Resident.certify (resident.py:133-136). */
+int pgj_resident_certify(const pgj_grid* g, int L, const double* a, double* u, const double* rhs, double* r,
+                         void* state, void* ws, void* stream);
+/* active = |r|^2 > target2 & iterations < 4000 & !bad; z = M^-1 r; first
+ * direction armed. Replaces Resident.initialize (resident.py:119-123). */
+int pgj_resident_init(const pgj_grid* g, int L, const void* plan, const double* s, const void* s_hat,
+                      const double* r, double* z, void* state, void* ws, void* stream);
+/* `count` (even) masked PCG iterations; p0/p1 are ping-pong direction
+ * buffers, q and z are scratch vectors, all (L, 3N). Replaces
+ * Resident.iteration with update_ur/update_p (resident.py:39-58, 125-131). */
+int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* s,
+                         const void* s_hat, double* u, double* r, double* p0, double* p1, double* q, double* z,
+                         void* state, void* ws, int count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PGJ_H */

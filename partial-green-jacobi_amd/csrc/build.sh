@@ -1,0 +1,15 @@
+#!/bin/sh
+# Builds libpgj.so for gfx950 from the HIP sources in this directory.
+set -e
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=${1:-$HERE/../pgj/libpgj.so}
+JOBS=${JOBS:-4}
+FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Xclang -target-feature -Xclang -packed-fp32-ops"
+TMP=$(mktemp -d)
+trap 'rm -rf "$TMP"' EXIT
+for src in common hex8 fft resident; do
+  hipcc $FLAGS -c "$HERE/$src.hip" -o "$TMP/$src.o" &
+done
+wait
+hipcc --offload-arch=gfx950 -shared -o "$OUT.tmp" "$TMP"/common.o "$TMP"/hex8.o "$TMP"/fft.o "$TMP"/resident.o
+mv "$OUT.tmp" "$OUT"

@@ -1,0 +1,44 @@
+// Declarations shared by the FFT/Green kernels and the resident solver.
+#pragma once
+
+#include "layout.h"
+#include "pgj_internal.h"
+
+namespace pgj {
+
+// ------------------------------------------------------------------ arguments
+enum { kFxyScale = 0, kFxyPre = 1, kFxyPCG = 2 };
+enum { kFzScale = 0, kFzPre = 1, kFzInit = 2, kFzPCG = 3 };
+enum { kFxyiPre = 0, kFxyiPCG = 1 };
+
+struct FFTArgs {
+  int nz, ny, nx;
+  int64_t N;
+  FFTGeom f;
+  const double2* twx;
+  const double2* twy;
+  const double2* twz;
+  const double* sym;
+  const double* rin;     // fxy pre: input residual
+  double* u;             // fxy pcg
+  double* r;             // fxy pcg
+  const double* p;       // fxy pcg
+  const double* q;       // fxy pcg
+  const double* s;       // scale (nullable)
+  const double2* shat;   // spectrum of s (nullable)
+  double2* spec;         // spectrum buffer (L x 3 x nf)
+  double* zout;          // fxyi output
+  double* pr;            // fxy partials  [l][z][c][2] = (sum r^2, sum r)
+  double* pg;            // fz partials   [l][blk][4]  = (v.w, s.w_x, s.w_y, s.w_z)
+  const double* ppq;     // hex8 partials [l][blk]
+  int ppq_count;
+  StateView st;
+};
+
+FFTArgs fft_args(const pgj_grid& g, const void* plan);
+int check_fft(const pgj_grid* g);
+int launch_fxy_mode(FFTArgs& A, int L, int mode, hipStream_t s);
+int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s);
+int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s);
+
+}  // namespace pgj

@@ -1,0 +1,739 @@
+// HEX8 operator kernels: the row-march apply (drop-in and fused PCG form),
+// the independent thread-per-node reference apply, rhs, energies, C^H,
+// mean stress, nodal diagonal / partial Green-Jacobi scale and projection.
+#include <cmath>
+
+#include "hex8_kernels.h"
+
+namespace pgj {
+
+// Value of the direction at a node for the fused update p = z + beta p_old.
+template <int MODE>
+__device__ __forceinline__ double node_value(const RowArgs& A, int64_t idx, double beta, bool first) {
+  if (MODE == kModeApply) return A.u[idx];
+  const double z = A.u[idx];
+  return first ? z : fma(beta, A.pold[idx], z);
+}
+
+template <int MODE, int XPL, int R, bool CUBIC>
+__global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l = blockIdx.z;
+  const RowMarch& c = A.cfg;
+  const int W = c.W;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const int64_t base = int64_t(l) * 3 * A.N;
+  const int y0 = blockIdx.x * W * R, z0 = blockIdx.y * c.ZC;
+  const int xl = lane % c.nl;
+  const bool lane_on = lane < c.nl;
+
+  double beta = 0.0;
+  bool first = false;
+  if (MODE == kModePCG) {
+    if (!A.st.active[l]) return;
+    first = A.st.first[l] != 0;
+    if (!first) {
+      const double rz_new = wave_reduce_partials(A.pg + int64_t(l) * A.pg_stride, A.pg_count, 4);
+      const double rz = A.st.rz[l];
+      beta = rz_new / (rz != 0.0 ? rz : 1.0);
+    }
+  } else if (A.mask && !A.mask[l]) {
+    // Inactive load: the reference kernel leaves its pre-zeroed output untouched.
+    if (wave < W && lane_on) {
+      for (int j = 0; j < R; ++j)
+        for (int zi = 0; zi < c.ZC; ++zi)
+          for (int xi = 0; xi < XPL; ++xi) {
+            const int64_t node = (int64_t(z0 + zi) * ny + (y0 + wave * R + j)) * nx + xl * XPL + xi;
+            for (int d = 0; d < 3; ++d) A.out[base + 3 * node + d] = 0.0;
+          }
+    }
+    return;
+  }
+
+  const bool halo = wave == W;
+  const int ybase = halo ? y0 - 1 : y0 + wave * R;
+  const int rows = halo ? 1 : R;  // element rows computed by this wave
+
+  double nodeL[R + 1][XPL][3], nodeU[R + 1][XPL][3];
+  double carry[R][XPL][3];
+  double accL[R + 1][XPL][3], accU[R + 1][XPL][3];
+  double pq = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+#pragma unroll
+    for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) carry[j][xi][d] = 0.0;
+
+  auto load_plane = [&](int plane, double (&dst)[R + 1][XPL][3], bool write_p) {
+#pragma unroll
+    for (int j = 0; j <= R; ++j) {
+      if (j > rows) break;
+      const int y = wrapi(ybase + j, ny);
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi) {
+        const int64_t node = (int64_t(plane) * ny + y) * nx + xl * XPL + xi;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int64_t idx = base + 3 * node + d;
+          const double v = node_value<MODE>(A, idx, beta, first);
+          dst[j][xi][d] = v;
+          if (MODE == kModePCG && write_p && j < rows && !halo && lane_on) A.pnew[idx] = v;
+        }
+      }
+    }
+  };
+
+  const int src_right = (xl + 1) % c.nl;       // lane holding x + XPL
+  const int src_left = (xl + c.nl - 1) % c.nl;  // lane holding x - 1
+  double* slot = lds + (size_t)wave * 64 * 6 * XPL;
+
+  for (int li = 0; li <= c.ZC; ++li) {
+    const int zl = wrapi(z0 - 1 + li, nz), zu = wrapi(z0 + li, nz);
+    if (li == 0) {
+      load_plane(zl, nodeL, false);
+    } else {
+#pragma unroll
+      for (int j = 0; j <= R; ++j)
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) nodeL[j][xi][d] = nodeU[j][xi][d];
+    }
+    load_plane(zu, nodeU, li < c.ZC);
+#pragma unroll
+    for (int j = 0; j <= R; ++j)
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          accL[j][xi][d] = (j < R) ? carry[j][xi][d] : 0.0;
+          accU[j][xi][d] = 0.0;
+        }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (j >= rows) break;
+      const int y = wrapi(ybase + j, ny);
+      // Right neighbours (x + XPL) of the four node rows this element row touches.
+      double nb[2][2][3];
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          nb[oy][0][d] = __shfl(nodeL[j + oy][0][d], src_right, 64);
+          nb[oy][1][d] = __shfl(nodeU[j + oy][0][d], src_right, 64);
+        }
+      double spill[2][2][3];  // contributions of this lane's last element to x + 1 (next lane)
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi) {
+        const int x = xl * XPL + xi;
+        const double ae = A.a[(int64_t(zl) * ny + y) * nx + x];
+        double ue[24], fe[24];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+          const int ox = n & 1, oy = (n >> 1) & 1, oz = (n >> 2) & 1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            double v;
+            if (ox == 0 || xi + 1 < XPL) {
+              const int xs = xi + ox;
+              v = oz ? nodeU[j + oy][xs][d] : nodeL[j + oy][xs][d];
+            } else {
+              v = nb[oy][oz][d];
+            }
+            ue[3 * n + d] = v;
+          }
+        }
+        hex8_element<CUBIC>(ue, ae, A.coef, fe);
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+          const int ox = n & 1, oy = (n >> 1) & 1, oz = (n >> 2) & 1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double v = fe[3 * n + d];
+            if (ox == 0) {
+              if (oz) accU[j + oy][xi][d] += v; else accL[j + oy][xi][d] += v;
+            } else if (xi + 1 < XPL) {
+              if (oz) accU[j + oy][xi + 1][d] += v; else accL[j + oy][xi + 1][d] += v;
+            } else {
+              spill[oy][oz][d] = v;
+            }
+          }
+        }
+      }
+      // Contributions of the left neighbour lane's last element to this lane's first x.
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          accL[j + oy][0][d] += __shfl(spill[oy][0][d], src_left, 64);
+          accU[j + oy][0][d] += __shfl(spill[oy][1][d], src_left, 64);
+        }
+    }
+    // Export the contributions to the first node row of the next wave.
+#pragma unroll
+    for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        slot[(xi * 6 + d) * 64 + lane] = halo ? accL[1][xi][d] : accL[R][xi][d];
+        slot[(xi * 6 + 3 + d) * 64 + lane] = halo ? accU[1][xi][d] : accU[R][xi][d];
+      }
+    __syncthreads();
+    if (!halo) {
+      const double* from = lds + (size_t)(wave == 0 ? W : wave - 1) * 64 * 6 * XPL;
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          accL[0][xi][d] += from[(xi * 6 + d) * 64 + lane];
+          accU[0][xi][d] += from[(xi * 6 + 3 + d) * 64 + lane];
+        }
+      if (li >= 1 && lane_on) {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+#pragma unroll
+          for (int xi = 0; xi < XPL; ++xi) {
+            const int64_t node = (int64_t(zl) * ny + (ybase + j)) * nx + xl * XPL + xi;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              A.out[base + 3 * node + d] = accL[j][xi][d];
+              if (MODE == kModePCG) pq = fma(nodeL[j][xi][d], accL[j][xi][d], pq);
+            }
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) carry[j][xi][d] = accU[j][xi][d];
+    }
+    __syncthreads();
+  }
+  if (MODE == kModePCG) {
+    const double total = block_sum(pq, lds);
+    if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
+  }
+}
+
+template <bool CUBIC, bool CERT>
+__global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
+  __shared__ double red[8];
+  const int l = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t base = int64_t(l) * 3 * A.N;
+  double res2 = 0.0;
+  if (i < A.N) {
+    const int x = int(i % A.nx), y = int((i / A.nx) % A.ny), z = int(i / (int64_t(A.nx) * A.ny));
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int ex = wrapi(x - (o & 1), A.nx), ey = wrapi(y - ((o >> 1) & 1), A.ny),
+                ez = wrapi(z - ((o >> 2) & 1), A.nz);
+      double ue[24], fe[24];
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        const int nxp = wrapi(ex + (n & 1), A.nx), nyp = wrapi(ey + ((n >> 1) & 1), A.ny),
+                  nzp = wrapi(ez + ((n >> 2) & 1), A.nz);
+        const int64_t node = (int64_t(nzp) * A.ny + nyp) * A.nx + nxp;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ue[3 * n + d] = A.u[base + 3 * node + d];
+      }
+      const double ae = A.a[(int64_t(ez) * A.ny + ey) * A.nx + ex];
+      hex8_element<CUBIC>(ue, ae, A.coef, fe);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[d] += fe[3 * o + d];
+    }
+    for (int d = 0; d < 3; ++d) {
+      if (CERT) {
+        const double r = A.rhs[base + 3 * i + d] - acc[d];
+        A.out[base + 3 * i + d] = r;
+        res2 = fma(r, r, res2);
+      } else {
+        A.out[base + 3 * i + d] = acc[d];
+      }
+    }
+  }
+  if (CERT) {
+    const double t = block_sum(res2, red);
+    if (threadIdx.x == 0) A.pres[(int64_t(l) * gridDim.x + blockIdx.x) * 4 + 3] = t;
+  }
+}
+
+// Per-block component sums of a (L, 3N) vector into partial rows of 4 doubles.
+__global__ __launch_bounds__(256) void component_sums(const double* v, int64_t N, double* part) {
+  __shared__ double red[8];
+  const int l = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double s[3] = {0.0, 0.0, 0.0};
+  if (i < N)
+    for (int d = 0; d < 3; ++d) s[d] = v[int64_t(l) * 3 * N + 3 * i + d];
+  for (int d = 0; d < 3; ++d) {
+    const double t = block_sum(s[d], red);
+    if (threadIdx.x == 0) part[(int64_t(l) * gridDim.x + blockIdx.x) * 4 + d] = t;
+  }
+}
+
+// v -= mean per load/component (mean from component_sums partials).
+__global__ __launch_bounds__(256) void subtract_mean(double* v, int64_t N, const double* part, int nblk) {
+  const int l = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const double* p = part + int64_t(l) * nblk * 4;
+  double mean[3];
+  for (int d = 0; d < 3; ++d) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += p[b * 4 + d];
+    mean[d] = s / double(N);
+  }
+  if (i < N)
+    for (int d = 0; d < 3; ++d) v[int64_t(l) * 3 * N + 3 * i + d] -= mean[d];
+}
+
+// ------------------------------------------------------------------------ rhs
+// f_node = -sum_{8 elements} a_e V Bbar_a^T C0 eps with centroid gradients +-1/(4h).
+__global__ __launch_bounds__(256) void hex8_rhs_kernel(int nz, int ny, int nx, ElementCoef c, const double* a,
+                                                       double m0, double m1, double m2, double m3, double m4,
+                                                       double m5, double* out) {
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
+  // C0 eps (Voigt, engineering shear)
+  const double tr = m0 + m1 + m2;
+  const double s0 = c.lam * tr + 2 * c.mu * m0, s1 = c.lam * tr + 2 * c.mu * m1, s2 = c.lam * tr + 2 * c.mu * m2;
+  const double s3 = c.mu * m3, s4 = c.mu * m4, s5 = c.mu * m5;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int o = 0; o < 8; ++o) {
+    const int ex = wrapi(x - (o & 1), nx), ey = wrapi(y - ((o >> 1) & 1), ny), ez = wrapi(z - ((o >> 2) & 1), nz);
+    const double ae = a[(int64_t(ez) * ny + ey) * nx + ex] * c.vol;
+    const double dx = (o & 1) ? c.g[0] : -c.g[0];
+    const double dy = (o & 2) ? c.g[1] : -c.g[1];
+    const double dz = (o & 4) ? c.g[2] : -c.g[2];
+    f[0] -= ae * (dx * s0 + dy * s5 + dz * s4);
+    f[1] -= ae * (dx * s5 + dy * s1 + dz * s3);
+    f[2] -= ae * (dx * s4 + dy * s3 + dz * s2);
+  }
+  for (int d = 0; d < 3; ++d) out[3 * i + d] = f[d];
+}
+
+// Affine element displacement chi_n,d = (E * (offset_n * h))_d for a macro strain.
+__device__ __forceinline__ void affine_local(const double* m, const ElementCoef& c, double (&chi)[24]) {
+  const double h[3] = {1.0 / (4.0 * c.g[0]), 1.0 / (4.0 * c.g[1]), 1.0 / (4.0 * c.g[2])};
+  const double E[3][3] = {{m[0], 0.5 * m[5], 0.5 * m[4]}, {0.5 * m[5], m[1], 0.5 * m[3]}, {0.5 * m[4], 0.5 * m[3], m[2]}};
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const double p[3] = {(n & 1) * h[0], ((n >> 1) & 1) * h[1], ((n >> 2) & 1) * h[2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) chi[3 * n + d] = E[d][0] * p[0] + E[d][1] * p[1] + E[d][2] * p[2];
+  }
+}
+
+struct MacroSet {
+  double m[6][6];
+};
+
+struct DiagTable {
+  double v[8][3];
+};
+
+__device__ __forceinline__ void gather_element(const double* u, int nz, int ny, int nx, int x, int y, int z,
+                                               double (&ue)[24]) {
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int xx = wrapi(x + (n & 1), nx), yy = wrapi(y + ((n >> 1) & 1), ny), zz = wrapi(z + ((n >> 2) & 1), nz);
+    const int64_t node = (int64_t(zz) * ny + yy) * nx + xx;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[3 * node + d];
+  }
+}
+
+template <bool CUBIC>
+__global__ __launch_bounds__(256) void hex8_energy_kernel(int nz, int ny, int nx, ElementCoef c, const double* u,
+                                                          MacroSet ms, double* out) {
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  if (e >= N) return;
+  const int x = int(e % nx), y = int((e / nx) % ny), z = int(e / (int64_t(nx) * ny));
+  double ue[24], fe[24], chi[24];
+  gather_element(u + int64_t(l) * 3 * N, nz, ny, nx, x, y, z, ue);
+  affine_local(ms.m[l], c, chi);
+#pragma unroll
+  for (int k = 0; k < 24; ++k) ue[k] += chi[k];
+  hex8_element<CUBIC>(ue, 1.0, c, fe);
+  double en = 0.0;
+#pragma unroll
+  for (int k = 0; k < 24; ++k) en = fma(ue[k], fe[k], en);
+  out[int64_t(l) * N + e] = en / c.vol;
+}
+
+// C^H partials: per block the 21 upper-triangle sums of a_e (Ke v_i) . v_j / V.
+template <bool CUBIC>
+__global__ __launch_bounds__(256) void effective_partials(int nz, int ny, int nx, ElementCoef c, const double* a,
+                                                          const double* u6, double* part) {
+  __shared__ double red[21][4];
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double s[21];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) s[k] = 0.0;
+  if (e < N) {
+    const int x = int(e % nx), y = int((e / nx) % ny), z = int(e / (int64_t(nx) * ny));
+    const double ae = a[e];
+    int idx = 0;
+    for (int i = 0; i < 6; ++i) {
+      double m[6] = {0, 0, 0, 0, 0, 0};
+      m[i] = 1.0;
+      double vi[24], fi[24], chi[24];
+      gather_element(u6 + int64_t(i) * 3 * N, nz, ny, nx, x, y, z, vi);
+      affine_local(m, c, chi);
+#pragma unroll
+      for (int k = 0; k < 24; ++k) vi[k] += chi[k];
+      hex8_element<CUBIC>(vi, 1.0, c, fi);
+      for (int j = i; j < 6; ++j, ++idx) {
+        double vj[24];
+        double mj[6] = {0, 0, 0, 0, 0, 0};
+        mj[j] = 1.0;
+        gather_element(u6 + int64_t(j) * 3 * N, nz, ny, nx, x, y, z, vj);
+        affine_local(mj, c, chi);
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 24; ++k) t = fma(fi[k], vj[k] + chi[k], t);
+        s[idx] = ae * t / c.vol;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 21; ++k) {
+    const double t = wave_sum(s[k]);
+    if (lane == 0) red[k][wave] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 21) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += red[threadIdx.x][w];
+    part[int64_t(blockIdx.x) * 21 + threadIdx.x] = t;
+  }
+}
+
+__global__ void effective_finish(const double* part, int nblk, int64_t N, double* out) {
+  const int idx = threadIdx.x;
+  if (idx >= 21) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[int64_t(b) * 21 + idx];
+  int k = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j, ++k)
+      if (k == idx) {
+        out[i * 6 + j] = s / double(N);
+        out[j * 6 + i] = s / double(N);
+      }
+}
+
+__global__ __launch_bounds__(256) void avg_stress_partials(int nz, int ny, int nx, ElementCoef c, const double* a,
+                                                           const double* u, MacroSet ms, double* part) {
+  __shared__ double red[8];
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double sig[6] = {0, 0, 0, 0, 0, 0};
+  if (e < N) {
+    const int x = int(e % nx), y = int((e / nx) % ny), z = int(e / (int64_t(nx) * ny));
+    double eps[6];
+    for (int k = 0; k < 6; ++k) eps[k] = ms.m[0][k];
+    for (int n = 0; n < 8; ++n) {
+      const int xx = wrapi(x + (n & 1), nx), yy = wrapi(y + ((n >> 1) & 1), ny), zz = wrapi(z + ((n >> 2) & 1), nz);
+      const int64_t node = (int64_t(zz) * ny + yy) * nx + xx;
+      const double ux = u[3 * node], uy = u[3 * node + 1], uz = u[3 * node + 2];
+      const double dx = (n & 1) ? c.g[0] : -c.g[0], dy = (n & 2) ? c.g[1] : -c.g[1], dz = (n & 4) ? c.g[2] : -c.g[2];
+      eps[0] += dx * ux;
+      eps[1] += dy * uy;
+      eps[2] += dz * uz;
+      eps[3] += dz * uy + dy * uz;
+      eps[4] += dz * ux + dx * uz;
+      eps[5] += dy * ux + dx * uy;
+    }
+    const double ae = a[e];
+    const double tr = eps[0] + eps[1] + eps[2];
+    for (int k = 0; k < 3; ++k) sig[k] = ae * (c.lam * tr + 2 * c.mu * eps[k]);
+    for (int k = 3; k < 6; ++k) sig[k] = ae * c.mu * eps[k];
+  }
+  for (int k = 0; k < 6; ++k) {
+    const double t = block_sum(sig[k], red);
+    if (threadIdx.x == 0) part[int64_t(blockIdx.x) * 6 + k] = t;
+  }
+}
+
+__global__ void avg_stress_finish(const double* part, int nblk, int64_t N, double* out) {
+  const int k = threadIdx.x;
+  if (k >= 6) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[int64_t(b) * 6 + k];
+  out[k] = s / double(N);
+}
+
+// Nodal diagonal: d_i = sum_o a_{i-o} diag(Ke)[o]; log partials for the scale.
+__global__ __launch_bounds__(256) void diagonal_kernel(int nz, int ny, int nx, const double* a, DiagTable ed,
+                                                       double* d, double* logpart) {
+  __shared__ double red[8];
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double lsum = 0.0;
+  if (i < N) {
+    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int o = 0; o < 8; ++o) {
+      const int ex = wrapi(x - (o & 1), nx), ey = wrapi(y - ((o >> 1) & 1), ny), ez = wrapi(z - ((o >> 2) & 1), nz);
+      const double ae = a[(int64_t(ez) * ny + ey) * nx + ex];
+      for (int k = 0; k < 3; ++k) acc[k] += ae * ed.v[o][k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      if (d) d[3 * i + k] = acc[k];
+      lsum += log(acc[k]);
+    }
+  }
+  const double t = block_sum(lsum, red);
+  if (threadIdx.x == 0) logpart[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(int nz, int ny, int nx, const double* a, DiagTable ed,
+                                                    double alpha, const double* logpart, int nblk, double* s) {
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double tot = 0.0;
+  for (int b = 0; b < nblk; ++b) tot += logpart[b];
+  const double mean = tot / double(3 * N);
+  if (i >= N) return;
+  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int o = 0; o < 8; ++o) {
+    const int ex = wrapi(x - (o & 1), nx), ey = wrapi(y - ((o >> 1) & 1), ny), ez = wrapi(z - ((o >> 2) & 1), nz);
+    const double ae = a[(int64_t(ez) * ny + ey) * nx + ex];
+    for (int k = 0; k < 3; ++k) acc[k] += ae * ed.v[o][k];
+  }
+  for (int k = 0; k < 3; ++k) s[3 * i + k] = exp(-0.5 * alpha * (log(acc[k]) - mean));
+}
+
+
+// ------------------------------------------------------------------ launchers
+
+bool cubic(const pgj_grid& g) { return g.hx == g.hy && g.hy == g.hz; }
+
+ElementCoef coef_of(const pgj_grid& g) { return make_coef(g.hx, g.hy, g.hz, g.lam, g.mu); }
+
+DiagTable diag_table(const pgj_grid& g) {
+  DiagTable t{};
+  ElementCoef c = coef_of(g);
+  for (int k = 0; k < 24; ++k) {
+    double u[24] = {0}, f[24];
+    u[k] = 1.0;
+    hex8_element<false>(u, 1.0, c, f);
+    t.v[k / 3][k % 3] = f[k];
+  }
+  return t;
+}
+
+int check_grid(const pgj_grid* g) {
+  PGJ_REQUIRE(g != nullptr, "null grid");
+  PGJ_REQUIRE(g->nx >= 2 && g->ny >= 2 && g->nz >= 2, "grid dimensions must be >= 2, got (%d,%d,%d)", g->nz, g->ny,
+              g->nx);
+  PGJ_REQUIRE(g->hx > 0 && g->hy > 0 && g->hz > 0, "positive voxel sizes required");
+  PGJ_REQUIRE(int64_t(g->nx) * g->ny * g->nz < (int64_t(1) << 31) / 3, "grid too large");
+  return PGJ_OK;
+}
+
+template <int MODE, int XPL, int R>
+int launch_rowmarch_t(const pgj_grid& g, int L, RowArgs& A, hipStream_t s) {
+  const RowMarch& c = A.cfg;
+  dim3 grid(c.by, c.bz, L), block(64 * (c.W + 1));
+  const size_t lds = size_t(c.W + 1) * 64 * 6 * XPL * 8;
+  if (cubic(g))
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true>), grid, block, lds, s, A);
+  else
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false>), grid, block, lds, s, A);
+  PGJ_LAUNCHED("hex8_rowmarch");
+  return PGJ_OK;
+}
+
+int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgs& A, hipStream_t s) {
+  A.nz = g.nz; A.ny = g.ny; A.nx = g.nx;
+  A.N = int64_t(g.nx) * g.ny * g.nz;
+  A.cfg = rowmarch_config(g);
+  A.coef = coef_of(g);
+  const RowMarch& c = A.cfg;
+  if (mode == kModeApply) {
+    if (c.xpl == 1) return c.R == 2 ? launch_rowmarch_t<kModeApply, 1, 2>(g, L, A, s) : launch_rowmarch_t<kModeApply, 1, 1>(g, L, A, s);
+    return c.R == 2 ? launch_rowmarch_t<kModeApply, 2, 2>(g, L, A, s) : launch_rowmarch_t<kModeApply, 2, 1>(g, L, A, s);
+  }
+  if (c.xpl == 1) return c.R == 2 ? launch_rowmarch_t<kModePCG, 1, 2>(g, L, A, s) : launch_rowmarch_t<kModePCG, 1, 1>(g, L, A, s);
+  return c.R == 2 ? launch_rowmarch_t<kModePCG, 2, 2>(g, L, A, s) : launch_rowmarch_t<kModePCG, 2, 1>(g, L, A, s);
+}
+
+int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_t s) {
+  A.nz = g.nz; A.ny = g.ny; A.nx = g.nx;
+  A.N = int64_t(g.nx) * g.ny * g.nz;
+  A.coef = coef_of(g);
+  dim3 grid(unsigned((A.N + 255) / 256), L), block(256);
+  const bool cb = cubic(g);
+  if (cert) {
+    if (cb) hipLaunchKernelGGL((hex8_reference<true, true>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((hex8_reference<false, true>), grid, block, 0, s, A);
+  } else {
+    if (cb) hipLaunchKernelGGL((hex8_reference<true, false>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((hex8_reference<false, false>), grid, block, 0, s, A);
+  }
+  PGJ_LAUNCHED("hex8_reference");
+  return PGJ_OK;
+}
+
+int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, hipStream_t s) {
+  const int64_t N = int64_t(g.nx) * g.ny * g.nz;
+  const int nblk = int((N + 255) / 256);
+  hipLaunchKernelGGL(subtract_mean, dim3(nblk, L), dim3(256), 0, s, v, N, part, nblk);
+  PGJ_LAUNCHED("subtract_mean");
+  return PGJ_OK;
+}
+
+int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, hipStream_t s) {
+  const int64_t N = int64_t(g.nx) * g.ny * g.nz;
+  dim3 grid(unsigned((N + 255) / 256), L);
+  hipLaunchKernelGGL(component_sums, grid, dim3(256), 0, s, v, N, part);
+  PGJ_LAUNCHED("component_sums");
+  return PGJ_OK;
+}
+
+}  // namespace pgj
+
+using namespace pgj;
+
+extern "C" int pgj_hex8_apply(const pgj_grid* g, int L, const double* a, const double* u, const uint8_t* active,
+                              double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && a && u && out, "invalid apply arguments");
+  if (!rowmarch_config(*g).valid) {
+    PGJ_REQUIRE(active == nullptr, "active mask needs the row-march kernel (nx <= 128 and nx %% ceil(nx/64) == 0)");
+    return pgj_hex8_apply_reference(g, L, a, u, out, stream);
+  }
+  RowArgs A{};
+  A.a = a; A.u = u; A.out = out; A.mask = active;
+  return launch_rowmarch(*g, L, kModeApply, A, as_stream(stream));
+}
+
+extern "C" int pgj_hex8_apply_reference(const pgj_grid* g, int L, const double* a, const double* u, double* out,
+                                        void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && a && u && out, "invalid apply arguments");
+  RefArgs A{};
+  A.a = a; A.u = u; A.out = out;
+  return launch_reference(*g, L, A, false, as_stream(stream));
+}
+
+extern "C" int pgj_hex8_rhs(const pgj_grid* g, const double* a, const double macro[6], double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(a && macro && out, "invalid rhs arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  hipLaunchKernelGGL(hex8_rhs_kernel, dim3(unsigned((N + 255) / 256)), dim3(256), 0, as_stream(stream), g->nz, g->ny,
+                     g->nx, coef_of(*g), a, macro[0], macro[1], macro[2], macro[3], macro[4], macro[5], out);
+  PGJ_LAUNCHED("hex8_rhs");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_hex8_energy(const pgj_grid* g, int L, const double* u, const double* macros, double* out,
+                               void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 6 && u && macros && out, "energy needs 1 <= L <= 6 loads");
+  MacroSet ms{};
+  for (int l = 0; l < L; ++l)
+    for (int k = 0; k < 6; ++k) ms.m[l][k] = macros[6 * l + k];
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  dim3 grid(unsigned((N + 255) / 256), L);
+  if (cubic(*g))
+    hipLaunchKernelGGL(hex8_energy_kernel<true>, grid, dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,
+                       coef_of(*g), u, ms, out);
+  else
+    hipLaunchKernelGGL(hex8_energy_kernel<false>, grid, dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,
+                       coef_of(*g), u, ms, out);
+  PGJ_LAUNCHED("hex8_energy");
+  return PGJ_OK;
+}
+
+extern "C" size_t pgj_effective_tensor_ws_bytes(const pgj_grid* g) {
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  return align256(size_t((N + 255) / 256) * 21 * 8);
+}
+
+extern "C" int pgj_effective_tensor(const pgj_grid* g, const double* a, const double* u6, double* ws, double* out36,
+                                    void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(a && u6 && ws && out36, "invalid effective tensor arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  const int nblk = int((N + 255) / 256);
+  hipStream_t s = as_stream(stream);
+  if (cubic(*g))
+    hipLaunchKernelGGL(effective_partials<true>, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a, u6, ws);
+  else
+    hipLaunchKernelGGL(effective_partials<false>, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a, u6,
+                       ws);
+  PGJ_LAUNCHED("effective_partials");
+  hipLaunchKernelGGL(effective_finish, dim3(1), dim3(64), 0, s, ws, nblk, N, out36);
+  PGJ_LAUNCHED("effective_finish");
+  return PGJ_OK;
+}
+
+extern "C" size_t pgj_avg_stress_ws_bytes(const pgj_grid* g) {
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  return align256(size_t((N + 255) / 256) * 6 * 8);
+}
+
+extern "C" int pgj_hex8_avg_stress(const pgj_grid* g, const double* a, const double* u, const double macro[6],
+                                   double* ws, double* out6, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(a && u && macro && ws && out6, "invalid stress arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  const int nblk = int((N + 255) / 256);
+  MacroSet ms{};
+  for (int k = 0; k < 6; ++k) ms.m[0][k] = macro[k];
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(avg_stress_partials, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a, u, ms, ws);
+  PGJ_LAUNCHED("avg_stress_partials");
+  hipLaunchKernelGGL(avg_stress_finish, dim3(1), dim3(64), 0, s, ws, nblk, N, out6);
+  PGJ_LAUNCHED("avg_stress_finish");
+  return PGJ_OK;
+}
+
+extern "C" size_t pgj_scale_ws_bytes(const pgj_grid* g) {
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  return align256(size_t((N + 255) / 256) * 8);
+}
+
+extern "C" int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alpha, double* d, double* s_out,
+                                  double* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(a && s_out && ws, "invalid scale arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  const int nblk = int((N + 255) / 256);
+  DiagTable t = diag_table(*g);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(diagonal_kernel, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, a, t, d, ws);
+  PGJ_LAUNCHED("diagonal_kernel");
+  hipLaunchKernelGGL(scale_kernel, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, a, t, alpha, ws, nblk, s_out);
+  PGJ_LAUNCHED("scale_kernel");
+  return PGJ_OK;
+}
+
+extern "C" size_t pgj_project_ws_bytes(const pgj_grid* g, int L) {
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  return align256(size_t(L) * size_t((N + 255) / 256) * 4 * 8);
+}
+
+extern "C" int pgj_project(const pgj_grid* g, int L, double* v, double* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && v && ws, "invalid projection arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  const int nblk = int((N + 255) / 256);
+  hipStream_t s = as_stream(stream);
+  (void)nblk;
+  if (int e = launch_component_sums(*g, L, v, ws, s)) return e;
+  return launch_subtract_mean(*g, L, v, ws, s);
+}

@@ -1,0 +1,129 @@
+// Exact 2x2x2-Gauss Q1/HEX8 element action for an isotropic material a*C0(lambda, mu).
+//
+// Derivation (independent of any particular code):
+//   Node a of the element sits at local coordinates xi_i = 2*bit_i(a) - 1. The
+//   trilinear field of component d is u_d(xi) = 1/8 sum_m U_d[m] xi^m over the
+//   eight monomials m in {0,1}^3, with U_d[m] = sum_a prod_{i in m}(2 a_i - 1) u_d[a]
+//   (three 2-point butterflies). Its gradient is
+//     du_d/dx_j = g_j sum_{m : m_j = 1} U_d[m] xi^(m - e_j),  g_j = 1/(4 h_j),
+//   so every strain component is a combination of seven monomials t (t != 111).
+//   Distinct monomials are orthogonal under the element integral, and
+//   (1/8) int xi^(2t) = 3^-|t|, which the 2-point rule reproduces exactly. Hence
+//     Ke u = a V sum_t 3^-|t| (d eps_t / du)^T C0 eps_t,
+//   evaluated as: forward butterflies -> per-monomial isotropic stress
+//   sigma = k_t (lambda tr(eps) I + 2 mu eps) -> modal forces
+//   F_d[m] = sum_{j: m_j=1} g_j sigma_dj(m - e_j) -> transposed butterflies.
+// Element node a has offset (a&1, (a>>1)&1, (a>>2)&1) in (x, y, z), dof 3a+d.
+#pragma once
+
+#ifndef PGJ_HD
+#if defined(__HIPCC__)
+#define PGJ_HD __host__ __device__
+#else
+#define PGJ_HD
+#endif
+#endif
+
+namespace pgj {
+
+struct ElementCoef {
+  double g[3];      // 1/(4 h_j)
+  double lam, mu;   // isotropic C0
+  double vol;       // hx*hy*hz
+};
+
+// f = a * Ke(C0) * u for one element. u and f are 24 values, dof 3a+d.
+template <bool kCubic>
+PGJ_HD inline void hex8_element(const double (&u)[24], double a, const ElementCoef& c, double (&f)[24]) {
+  double U[3][8];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) U[d][n] = u[3 * n + d];
+  // Forward butterflies: (lo, hi) -> (lo + hi, hi - lo) along x, y, z.
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        if (!(n & b)) {
+          const double lo = U[d][n], hi = U[d][n | b];
+          U[d][n] = lo + hi;
+          U[d][n | b] = hi - lo;
+        }
+  // Gradient scale: with a cubic voxel g^2 is folded into k_t.
+  const double gs0 = kCubic ? 1.0 : c.g[0];
+  const double gs1 = kCubic ? 1.0 : c.g[1];
+  const double gs2 = kCubic ? 1.0 : c.g[2];
+  const double gs[3] = {gs0, gs1, gs2};
+  const double base = kCubic ? a * c.vol * c.g[0] * c.g[0] : a * c.vol;
+  double F[3][8];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) F[d][n] = 0.0;
+#pragma unroll
+  for (int t = 0; t < 7; ++t) {
+    const int bits = ((t & 1) ? 1 : 0) + ((t & 2) ? 1 : 0) + ((t & 4) ? 1 : 0);
+    const double w = bits == 0 ? 1.0 : (bits == 1 ? (1.0 / 3.0) : (1.0 / 9.0));
+    const double k = base * w;
+    // Gradient entries G[d][j] = g_j U_d[t | e_j], present iff bit j of t is clear.
+    double G[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G[d][j] = (t & (1 << j)) ? 0.0 : gs[j] * U[d][t | (1 << j)];
+    double tr = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (!(t & (1 << j))) tr += G[j][j];
+    const double ltr = c.lam * k * tr;
+    const double two_mu_k = 2.0 * c.mu * k;
+    const double mu_k = c.mu * k;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (!(t & (1 << j))) F[j][t | (1 << j)] += gs[j] * (ltr + two_mu_k * G[j][j]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = d + 1; j < 3; ++j) {
+        const bool jd = !(t & (1 << j));  // G[d][j] present
+        const bool dj = !(t & (1 << d));  // G[j][d] present
+        if (jd || dj) {
+          const double s = mu_k * ((jd ? G[d][j] : 0.0) + (dj ? G[j][d] : 0.0));
+          if (jd) F[d][t | (1 << j)] += gs[j] * s;
+          if (dj) F[j][t | (1 << d)] += gs[d] * s;
+        }
+      }
+  }
+  // Transposed butterflies: (lo, hi) -> (lo - hi, lo + hi) along z, y, x.
+#pragma unroll
+  for (int b = 4; b >= 1; b >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        if (!(n & b)) {
+          const double lo = F[d][n], hi = F[d][n | b];
+          F[d][n] = lo - hi;
+          F[d][n | b] = lo + hi;
+        }
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[3 * n + d] = F[d][n];
+}
+
+inline ElementCoef make_coef(double hx, double hy, double hz, double lam, double mu) {
+  ElementCoef c;
+  c.g[0] = 1.0 / (4.0 * hx);
+  c.g[1] = 1.0 / (4.0 * hy);
+  c.g[2] = 1.0 / (4.0 * hz);
+  c.lam = lam;
+  c.mu = mu;
+  c.vol = hx * hy * hz;
+  return c;
+}
+
+}  // namespace pgj

@@ -1,0 +1,56 @@
+// Declarations shared by the HEX8 kernels and the resident solver.
+#pragma once
+
+#include "hex8_element.h"
+#include "layout.h"
+#include "pgj_internal.h"
+
+namespace pgj {
+
+enum { kModeApply = 0, kModePCG = 1 };
+
+struct RowArgs {
+  int nz, ny, nx;
+  int64_t N;
+  RowMarch cfg;
+  ElementCoef coef;
+  const double* a;
+  const double* u;       // apply: input vectors; PCG: preconditioned residual z
+  double* out;           // apply: K u; PCG: q = K p_new
+  const uint8_t* mask;   // apply: optional active mask
+  // PCG fusion
+  const double* pold;
+  double* pnew;
+  StateView st;
+  const double* pg;      // FZ partials (rz part first of 4), per load stride pg_stride
+  int pg_count, pg_stride;
+  double* ppq;           // per-block p.q partials, per load stride ppq_stride
+  int ppq_stride;
+};
+
+// Independent reference apply: one thread per node gathers the rows of its
+// eight elements; optionally fused with the certification residual.
+struct RefArgs {
+  int nz, ny, nx;
+  int64_t N;
+  ElementCoef coef;
+  bool cubic;
+  const double* a;
+  const double* u;
+  double* out;
+  // certification residual: out = rhs - K u, per-block |r|^2 into slot 3 of
+  // 4-wide partial rows [l][block][4]
+  const double* rhs;
+  double* pres;
+};
+
+
+bool cubic(const pgj_grid& g);
+ElementCoef coef_of(const pgj_grid& g);
+int check_grid(const pgj_grid* g);
+int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgs& A, hipStream_t s);
+int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_t s);
+int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, hipStream_t s);
+int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, hipStream_t s);
+
+}  // namespace pgj

@@ -1,0 +1,256 @@
+// Launch configurations and the byte layouts of the caller-owned plan,
+// workspace and resident-state buffers. Host and device agree on these
+// through the structs below (they are passed to kernels by value).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/pgj.h"
+
+namespace pgj {
+
+constexpr int kCap = 4000;   // PCG iteration cap (exp/asg_gpu_resident_20261103/resident.py:50)
+
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+inline size_t align8(size_t v) { return (v + 7) & ~size_t(7); }
+
+// ---------------------------------------------------------------- row-march
+// The HEX8 row-march kernel maps x to lanes (XPL consecutive x per lane), a
+// wavefront marches R element rows over the z-layers of a block, a block has
+// W row-waves plus one halo wave and covers W*R node rows and ZC planes.
+struct RowMarch {
+  int valid;
+  int xpl, nl;      // x per lane, active lanes = nx / xpl
+  int R, W, ZC;
+  int by, bz;       // blocks along y and z per load
+};
+
+inline RowMarch rowmarch_config(const pgj_grid& g) {
+  RowMarch c{};
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  c.xpl = (nx + 63) / 64;
+  if (c.xpl > 2 || nx % c.xpl != 0 || ny < 2 || nz < 2) return c;
+  c.nl = nx / c.xpl;
+  c.R = 1;
+  const int rows = ny / c.R;
+  const int wmax = c.xpl == 1 ? 8 : 4;
+  c.W = 1;
+  for (int w = wmax; w >= 1; --w)
+    if (rows % w == 0) { c.W = w; break; }
+  c.by = ny / (c.W * c.R);
+  c.ZC = 1;
+  for (int z = 4; z >= 1; --z)
+    if (nz % z == 0) { c.ZC = z; break; }
+  c.bz = nz / c.ZC;
+  c.valid = 1;
+  return c;
+}
+
+inline int hex8_blocks_per_load(const pgj_grid& g) {
+  RowMarch c = rowmarch_config(g);
+  if (c.valid) return c.by * c.bz;
+  const int64_t n = int64_t(g.nx) * g.ny * g.nz;
+  return int((n + 255) / 256);
+}
+
+// --------------------------------------------------------------------- FFT
+// One-dimensional transforms of length n are factored into radices 4, 2, 3, 5, 7.
+constexpr int kMaxStages = 12;
+struct Factors {
+  int n, count;
+  int r[kMaxStages];
+};
+
+inline Factors factorize(int n) {
+  Factors f{};
+  f.n = n;
+  int m = n;
+  while (m % 4 == 0 && f.count < kMaxStages) { f.r[f.count++] = 4; m /= 4; }
+  while (m % 2 == 0 && f.count < kMaxStages) { f.r[f.count++] = 2; m /= 2; }
+  for (int p : {3, 5, 7})
+    while (m % p == 0 && f.count < kMaxStages) { f.r[f.count++] = p; m /= p; }
+  if (m != 1) f.count = -1;
+  return f;
+}
+
+struct FFTGeom {
+  int nz, ny, nx;
+  int nkx;          // nx/2 + 1
+  int M;            // complex length of the x transform: nx/2 (even nx) or nx (odd)
+  int rs;           // row stride of a plane row in LDS (complex): max(nkx, M)
+  int64_t nf;       // nz*ny*nkx
+  int comps;        // components per plane block in the xy kernels (3 or 1)
+  int kc;           // kx columns per block in the z kernel
+  int nkb;          // ceil(nkx/kc)
+  int threads_xy, threads_z;
+  size_t lds_xy, lds_z;
+  Factors fx, fy, fz;
+  int valid;
+};
+
+constexpr size_t kLdsLimit = 160 * 1024 - 2048;
+constexpr int kStageValues = 16;  // complex values a thread holds in one in-place LDS stage
+
+// True when every stage of a batched transform fits the register budget of
+// one in-place LDS stage (each thread holds kStageValues/R butterflies).
+inline bool stages_fit(const Factors& f, int lines, int threads) {
+  for (int s = 0; s < f.count; ++s) {
+    const int R = f.r[s];
+    if (int64_t(lines) * (f.n / R) > int64_t(threads) * (kStageValues / R)) return false;
+  }
+  return true;
+}
+
+inline FFTGeom fft_geom(const pgj_grid& g) {
+  FFTGeom f{};
+  f.nz = g.nz; f.ny = g.ny; f.nx = g.nx;
+  f.nkx = g.nx / 2 + 1;
+  f.M = (g.nx % 2 == 0) ? g.nx / 2 : g.nx;
+  f.rs = f.nkx > f.M ? f.nkx : f.M;
+  f.nf = int64_t(g.nz) * g.ny * f.nkx;
+  f.fx = factorize(f.M);
+  f.fy = factorize(g.ny);
+  f.fz = factorize(g.nz);
+  f.valid = f.fx.count >= 0 && f.fy.count >= 0 && f.fz.count >= 0 && g.nx >= 2 && g.ny >= 2 && g.nz >= 2 &&
+            g.nx <= 256 && g.ny <= 256 && g.nz <= 256;
+  const size_t plane = size_t(g.ny) * f.rs * 16;
+  f.comps = (3 * plane + 64 <= kLdsLimit) ? 3 : 1;
+  if (plane + 64 > kLdsLimit) f.valid = 0;
+  f.lds_xy = f.comps * plane + 64 * 8;
+  f.threads_xy = 512;
+  // z kernel: kc columns x 3 comps x nz complex
+  const size_t col = size_t(g.nz) * 3 * 16;
+  int kc = 1;
+  while (kc < f.nkx && size_t(kc + 1) * col + 64 * 8 <= 56 * 1024) ++kc;
+  f.kc = kc;
+  f.nkb = (f.nkx + kc - 1) / kc;
+  f.lds_z = size_t(kc) * col + 64 * 8;
+  f.threads_z = 256;
+  if (f.valid) {
+    const int rows = f.comps * g.ny;
+    f.valid = stages_fit(f.fx, rows, f.threads_xy) && stages_fit(f.fy, f.comps * f.nkx, f.threads_xy) &&
+              stages_fit(f.fz, 3 * f.kc, f.threads_z);
+  }
+  return f;
+}
+
+// Plan buffer: twiddles for x (length M), y, z; then the 6-coefficient symbol.
+struct PlanLayout {
+  size_t tw_x, tw_y, tw_z, sym, bytes;
+};
+
+inline PlanLayout plan_layout(const pgj_grid& g) {
+  FFTGeom f = fft_geom(g);
+  PlanLayout p{};
+  size_t off = 0;
+  p.tw_x = off; off = align256(off + size_t(g.nx) * 16);
+  p.tw_y = off; off = align256(off + size_t(g.ny) * 16);
+  p.tw_z = off; off = align256(off + size_t(g.nz) * 16);
+  p.sym = off; off = align256(off + size_t(6) * f.nf * 8);
+  p.bytes = off;
+  return p;
+}
+
+// Preconditioner workspace (also the Green part of the resident workspace).
+struct PrecondLayout {
+  size_t spec;      // L x 3 x nf complex
+  size_t pr;        // L x nz x 3 x 2 doubles  (per plane/component: sum r^2, sum r)
+  size_t pg;        // L x nkb*ny x 4 doubles  (rz part, sum s_c w_c)
+  size_t bytes;
+};
+
+inline PrecondLayout precond_layout(const pgj_grid& g, int L) {
+  FFTGeom f = fft_geom(g);
+  PrecondLayout p{};
+  size_t off = 0;
+  p.spec = off; off = align256(off + size_t(L) * 3 * f.nf * 16);
+  p.pr = off; off = align256(off + size_t(L) * g.nz * 6 * 8);
+  p.pg = off; off = align256(off + size_t(L) * f.nkb * g.ny * 4 * 8);
+  p.bytes = off;
+  return p;
+}
+
+// Resident workspace: preconditioner part + operator partials + certify partials.
+struct ResidentLayout {
+  PrecondLayout pre;
+  size_t ppq;       // L x hex8_blocks doubles
+  size_t pcert;     // L x nblk x 4 doubles (sum u_c, |r|^2)
+  int nblk_cert;
+  size_t bytes;
+};
+
+inline ResidentLayout resident_layout(const pgj_grid& g, int L) {
+  ResidentLayout r{};
+  r.pre = precond_layout(g, L);
+  size_t off = r.pre.bytes;
+  r.ppq = off; off = align256(off + size_t(L) * hex8_blocks_per_load(g) * 8);
+  const int64_t n = int64_t(g.nx) * g.ny * g.nz;
+  r.nblk_cert = int((n + 255) / 256);
+  r.pcert = off; off = align256(off + size_t(L) * r.nblk_cert * 4 * 8);
+  r.bytes = off;
+  return r;
+}
+
+// Resident state: public blocks first (documented in pgj.h), then private.
+struct StateLayout {
+  size_t rz, target2, norm, absolute, it, active, bad;
+  size_t it_next, active_next, bad_next, first, alpha;
+  size_t bytes;
+};
+
+inline StateLayout state_layout(int L) {
+  StateLayout s{};
+  size_t off = 0;
+  s.rz = off; off += align8(size_t(L) * 8);
+  s.target2 = off; off += align8(size_t(L) * 8);
+  s.norm = off; off += align8(size_t(L) * 8);
+  s.absolute = off; off += align8(size_t(L) * 8);
+  s.it = off; off += align8(size_t(L) * 8);
+  s.active = off; off += align8(size_t(L));
+  s.bad = off; off += align8(size_t(L));
+  s.it_next = off; off += align8(size_t(L) * 8);
+  s.active_next = off; off += align8(size_t(L));
+  s.bad_next = off; off += align8(size_t(L));
+  s.first = off; off += align8(size_t(L));
+  s.alpha = off; off += align8(size_t(L) * 8);
+  s.bytes = align256(off);
+  return s;
+}
+
+// Device view of the resident state (pointers into one buffer).
+struct StateView {
+  double* rz;
+  double* target2;
+  double* norm;
+  double* absolute;
+  int64_t* it;
+  uint8_t* active;
+  uint8_t* bad;
+  int64_t* it_next;
+  uint8_t* active_next;
+  uint8_t* bad_next;
+  uint8_t* first;
+  double* alpha;
+};
+
+inline StateView state_view(void* base, int L) {
+  StateLayout s = state_layout(L);
+  char* b = static_cast<char*>(base);
+  StateView v;
+  v.rz = reinterpret_cast<double*>(b + s.rz);
+  v.target2 = reinterpret_cast<double*>(b + s.target2);
+  v.norm = reinterpret_cast<double*>(b + s.norm);
+  v.absolute = reinterpret_cast<double*>(b + s.absolute);
+  v.it = reinterpret_cast<int64_t*>(b + s.it);
+  v.active = reinterpret_cast<uint8_t*>(b + s.active);
+  v.bad = reinterpret_cast<uint8_t*>(b + s.bad);
+  v.it_next = reinterpret_cast<int64_t*>(b + s.it_next);
+  v.active_next = reinterpret_cast<uint8_t*>(b + s.active_next);
+  v.bad_next = reinterpret_cast<uint8_t*>(b + s.bad_next);
+  v.first = reinterpret_cast<uint8_t*>(b + s.first);
+  v.alpha = reinterpret_cast<double*>(b + s.alpha);
+  return v;
+}
+
+}  // namespace pgj

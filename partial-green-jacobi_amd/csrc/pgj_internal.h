@@ -1,0 +1,70 @@
+// Shared helpers for the libpgj HIP sources: error reporting, launch checks,
+// index arithmetic and wavefront/block reductions with a fixed summation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/pgj.h"
+
+namespace pgj {
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define PGJ_REQUIRE(cond, ...)          \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::pgj::set_error(__VA_ARGS__);    \
+      return PGJ_ERR_ARG;               \
+    }                                   \
+  } while (0)
+
+#define PGJ_LAUNCHED(name)                                                        \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess) {                                                       \
+      ::pgj::set_error("%s launch failed: %s", name, hipGetErrorString(e_));      \
+      return PGJ_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Sum over the 64 lanes of a wavefront with a fixed butterfly order; every lane gets the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum with fixed order (lanes, then waves in index order). `scratch`
+// must hold blockDim.x/64 doubles. All threads receive the total.
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; ++w) t += scratch[w];
+  __syncthreads();
+  return t;
+}
+
+// Deterministic reduction of `count` partials at `p` (stride `stride`) by one
+// wavefront: lane-strided accumulation followed by wave_sum. Every lane of
+// every calling wavefront obtains bit-identical results.
+__device__ __forceinline__ double wave_reduce_partials(const double* p, int count, int stride) {
+  const int lane = threadIdx.x & 63;
+  double v = 0.0;
+  for (int i = lane; i < count; i += 64) v += p[(int64_t)i * stride];
+  return wave_sum(v);
+}
+
+}  // namespace pgj

@@ -1,0 +1,169 @@
+// Resident multi-load PCG: set-up, certification, initialization and the
+// fused iteration (four kernels per iteration, no host round trip):
+//   1. hex8_rowmarch<PCG>: p = z + beta p (or z right after init), q = K p, p.q partials
+//   2. fxy<PCG>          : alpha = rz / p.q; u += alpha p; r -= alpha q; forward xy FFT of s*r
+//   3. fz<PCG>           : masks, latches, iteration count; z-FFT, projection, Green, Parseval partials
+//   4. fxyi<PCG>         : inverse FFT, z = P(s*w); publishes the new active mask
+// Per-load scalar semantics follow exp/asg_gpu_resident_20261103/resident.py:39-58 and 119-136.
+#include <cmath>
+
+#include "fft_kernels.h"
+#include "hex8_kernels.h"
+
+namespace pgj {
+
+__global__ __launch_bounds__(256) void square_partials(const double* v, int64_t N, double* part) {
+  __shared__ double red[8];
+  const int l = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < N)
+    for (int d = 0; d < 3; ++d) {
+      const double x = v[int64_t(l) * 3 * N + 3 * i + d];
+      s = fma(x, x, s);
+    }
+  const double t = block_sum(s, red);
+  if (threadIdx.x == 0) part[(int64_t(l) * gridDim.x + blockIdx.x) * 4 + 3] = t;
+}
+
+__global__ void setup_finish(const double* part, int nblk, double rtol, double atol, StateView st) {
+  const int l = blockIdx.x;
+  const double s = wave_reduce_partials(part + int64_t(l) * nblk * 4 + 3, nblk, 4);
+  if (threadIdx.x == 0) {
+    const double norm = sqrt(s);
+    const double t = fmax(norm * rtol, atol);
+    st.norm[l] = norm;
+    st.target2[l] = t * t;
+    st.it[l] = 0;
+    st.bad[l] = 0;
+    st.active[l] = 0;
+    st.first[l] = 0;
+  }
+}
+
+__global__ void certify_finish(const double* part, int nblk, StateView st) {
+  const int l = blockIdx.x;
+  const double s = wave_reduce_partials(part + int64_t(l) * nblk * 4 + 3, nblk, 4);
+  if (threadIdx.x == 0) st.absolute[l] = sqrt(s);
+}
+
+struct ResidentBuffers {
+  ResidentLayout lay;
+  double* ppq;
+  double* pcert;
+  double2* spec;
+  double* pr;
+  double* pg;
+};
+
+ResidentBuffers resident_buffers(const pgj_grid& g, int L, void* ws) {
+  ResidentBuffers b{};
+  b.lay = resident_layout(g, L);
+  char* w = static_cast<char*>(ws);
+  b.ppq = reinterpret_cast<double*>(w + b.lay.ppq);
+  b.pcert = reinterpret_cast<double*>(w + b.lay.pcert);
+  b.spec = reinterpret_cast<double2*>(w + b.lay.pre.spec);
+  b.pr = reinterpret_cast<double*>(w + b.lay.pre.pr);
+  b.pg = reinterpret_cast<double*>(w + b.lay.pre.pg);
+  return b;
+}
+
+}  // namespace pgj
+
+using namespace pgj;
+
+extern "C" size_t pgj_resident_state_bytes(int L) { return state_layout(L).bytes; }
+
+extern "C" int pgj_resident_state_offsets(int L, size_t* o) {
+  PGJ_REQUIRE(L >= 1 && o, "invalid state layout query");
+  StateLayout s = state_layout(L);
+  o[0] = s.rz; o[1] = s.target2; o[2] = s.norm; o[3] = s.absolute; o[4] = s.it; o[5] = s.active; o[6] = s.bad;
+  return PGJ_OK;
+}
+
+extern "C" size_t pgj_resident_ws_bytes(const pgj_grid* g, int L) { return resident_layout(*g, L).bytes; }
+
+extern "C" int pgj_resident_setup(const pgj_grid* g, int L, const double* a, const double* macros, double rtol,
+                                  double* rhs, void* state, void* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 64 && a && macros && rhs && state && ws, "invalid resident setup arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  hipStream_t s = as_stream(stream);
+  for (int l = 0; l < L; ++l)
+    if (int e = pgj_hex8_rhs(g, a, macros + 6 * l, rhs + int64_t(l) * 3 * N, stream)) return e;
+  ResidentBuffers b = resident_buffers(*g, L, ws);
+  const int nblk = b.lay.nblk_cert;
+  hipLaunchKernelGGL(square_partials, dim3(nblk, L), dim3(256), 0, s, rhs, N, b.pcert);
+  PGJ_LAUNCHED("square_partials");
+  hipLaunchKernelGGL(setup_finish, dim3(L), dim3(64), 0, s, b.pcert, nblk, rtol, 1e-13 / sqrt(double(g->nx)),
+                     state_view(state, L));
+  PGJ_LAUNCHED("setup_finish");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_resident_certify(const pgj_grid* g, int L, const double* a, double* u, const double* rhs,
+                                    double* r, void* state, void* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 64 && a && u && rhs && r && state && ws, "invalid certify arguments");
+  hipStream_t s = as_stream(stream);
+  ResidentBuffers b = resident_buffers(*g, L, ws);
+  if (int e = launch_component_sums(*g, L, u, b.pcert, s)) return e;
+  if (int e = launch_subtract_mean(*g, L, u, b.pcert, s)) return e;
+  RefArgs A{};
+  A.a = a; A.u = u; A.out = r; A.rhs = rhs; A.pres = b.pcert;
+  if (int e = launch_reference(*g, L, A, true, s)) return e;
+  hipLaunchKernelGGL(certify_finish, dim3(L), dim3(64), 0, s, b.pcert, b.lay.nblk_cert, state_view(state, L));
+  PGJ_LAUNCHED("certify_finish");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, const double* sc, const void* s_hat,
+                                 const double* r, double* z, void* state, void* ws, void* stream) {
+  if (int e = check_fft(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 64 && plan && r && z && state && ws, "invalid init arguments");
+  PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
+  hipStream_t s = as_stream(stream);
+  ResidentBuffers b = resident_buffers(*g, L, ws);
+  FFTArgs A = fft_args(*g, plan);
+  A.rin = r; A.s = sc; A.shat = static_cast<const double2*>(s_hat);
+  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z;
+  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g);
+  A.st = state_view(state, L);
+  if (int e = launch_fxy_mode(A, L, kFxyPre, s)) return e;
+  if (int e = launch_fz_mode(A, L, kFzInit, s)) return e;
+  return launch_fxyi_mode(A, L, kFxyiPCG, s);
+}
+
+extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
+                                    const void* s_hat, double* u, double* r, double* p0, double* p1, double* q,
+                                    double* z, void* state, void* ws, int count, void* stream) {
+  if (int e = check_fft(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 64 && plan && a && u && r && p0 && p1 && q && z && state && ws,
+              "invalid iterate arguments");
+  PGJ_REQUIRE(count >= 2 && count % 2 == 0, "iteration count must be even and >= 2, got %d", count);
+  PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
+  PGJ_REQUIRE(rowmarch_config(*g).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
+  hipStream_t s = as_stream(stream);
+  ResidentBuffers b = resident_buffers(*g, L, ws);
+  StateView st = state_view(state, L);
+  FFTArgs F = fft_args(*g, plan);
+  F.u = u; F.r = r; F.q = q; F.s = sc; F.shat = static_cast<const double2*>(s_hat);
+  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z;
+  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g);
+  F.st = st;
+  const int pg_count = F.f.nkb * g->ny;
+  for (int i = 0; i < count; ++i) {
+    double* pold = (i % 2 == 0) ? p0 : p1;
+    double* pnew = (i % 2 == 0) ? p1 : p0;
+    RowArgs K{};
+    K.a = a; K.u = z; K.out = q; K.pold = pold; K.pnew = pnew; K.st = st;
+    K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
+    K.ppq = b.ppq; K.ppq_stride = F.ppq_count;
+    if (int e = launch_rowmarch(*g, L, kModePCG, K, s)) return e;
+    F.p = pnew;
+    if (int e = launch_fxy_mode(F, L, kFxyPCG, s)) return e;
+    if (int e = launch_fz_mode(F, L, kFzPCG, s)) return e;
+    if (int e = launch_fxyi_mode(F, L, kFxyiPCG, s)) return e;
+  }
+  return PGJ_OK;
+}

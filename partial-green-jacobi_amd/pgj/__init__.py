@@ -1,0 +1,10 @@
+"""pgj: MI355X-native partial Green-Jacobi homogenization hot path.
+
+Host-side mirror of the reference's operator / preconditioner / PCG / resident
+interfaces over the HIP C ABI in ``libpgj.so`` (declared in ``include/pgj.h``).
+"""
+from . import _lib
+from .operator import Operator, build, diagonal, scale, set_alpha
+from .resident import Resident
+
+__all__ = ['Operator', 'Resident', 'build', 'diagonal', 'scale', 'set_alpha']

@@ -1,0 +1,233 @@
+"""Resident multi-load FP64 PCG with hipGraph-captured iterations.
+
+Mirrors ``exp/asg_gpu_resident_20261103/resident.py:61-213`` (``Resident``)
+and the 6-load grouping/energy/C^H of
+``exp/asg_sctop_reproduction_20261105/reproduction.py:121-176``
+(``PaperResident``): per-load independent recurrences with device-latched
+masks, an iteration cap of 4000, up to two restarts from the certified
+residual, true-residual certification ``|f - K u| <= max(rtol |f|, 1e-13/sqrt(n))``
+and ``RuntimeError`` on failure. Only per-load scalars cross to the host, once
+per ``chunk`` iterations.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
+
+LOADS6 = np.eye(6)
+
+
+class Resident:
+    """Solves L unit-strain loads on one periodic grid, ``batch`` loads per recurrence group.
+
+    ``load_mode='serial'`` solves one load at a time (batch 1), ``'parallel'``
+    groups loads (batch 3 by default, as the reference; any divisor of L may
+    be given with ``batch``).
+    """
+
+    def __init__(self, n, loads=LOADS6, load_mode='serial', batch=None, chunk=4, lam=LAMBDA_STUDY2,
+                 mu=MU_STUDY2, device='cuda'):
+        if load_mode not in ('serial', 'parallel'):
+            raise ValueError('load_mode must be serial or parallel')
+        if chunk < 2 or chunk % 2:
+            raise ValueError('graph chunk must be even and >= 2')
+        self.loads = np.atleast_2d(np.asarray(loads, dtype=np.float64))
+        nl = len(self.loads)
+        self.batch = 1 if load_mode == 'serial' else (batch or min(3, nl))
+        if nl % self.batch:
+            raise ValueError('batch must divide the number of loads')
+        self.load_mode, self.n, self.chunk = load_mode, int(n), chunk
+        shape = (n, n, n) if np.isscalar(n) else tuple(n)
+        self.shape = shape
+        self.op = Operator(torch.ones(shape, dtype=torch.float64, device=device), None, lam, mu, device).enable_green()
+        g = self.op.grid
+        N = int(np.prod(shape))
+        B = self.batch
+        dev = torch.device(device)
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        self.u, self.r, self.p0, self.p1, self.q, self.zv, self.rhs = (z(B, 3 * N) for _ in range(7))
+        self.solutions = z(nl, 3 * N)
+        lib = _lib.lib()
+        self.state = torch.zeros(lib.pgj_resident_state_bytes(B), dtype=torch.uint8, device=dev)
+        offs = (_lib.c_size_t * 7)()
+        _lib.check(lib.pgj_resident_state_offsets(B, offs))
+        o = list(offs)
+        self.rz = self.state[o[0]:o[0] + 8 * B].view(torch.float64)
+        self.target2 = self.state[o[1]:o[1] + 8 * B].view(torch.float64)
+        self.norm = self.state[o[2]:o[2] + 8 * B].view(torch.float64)
+        self.absolute = self.state[o[3]:o[3] + 8 * B].view(torch.float64)
+        self.iterations = self.state[o[4]:o[4] + 8 * B].view(torch.int64)
+        self.active = self.state[o[5]:o[5] + B]
+        self.bad = self.state[o[6]:o[6] + B]
+        self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, B), dtype=torch.uint8, device=dev)
+        self.scale = torch.ones((1, 3 * N), dtype=torch.float64, device=dev)
+        self.scale_hat = self.op.scale_spectrum(self.scale)
+        self.alpha = torch.zeros((), dtype=torch.float64, device=dev)
+        self.energy_out = z(nl, N)
+        self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
+        self.ch_out = z(36)
+        self.init_graph = self.iter_graph = self.cert_graph = None
+
+    # -- kernels ---------------------------------------------------------------------
+    def K(self, u, active=None):
+        """Scalar-a operator for (B, 3N) vectors (``resident.py:98``)."""
+        return self.op.apply_batched(u, active)
+
+    def true_K(self, u):
+        """Certification operator: the independent reference kernel (``resident.py:100-102``)."""
+        return self.op.apply_reference(u)
+
+    def B(self, r):
+        """P s G s P r (``resident.py:104-117``)."""
+        return self.op.green_precond(r, self.scale, self.scale_hat)
+
+    def _setup(self, load_indices, rtol):
+        m = self.loads[list(load_indices)].reshape(-1)
+        _lib.call('pgj_resident_setup', self.op.grid, len(load_indices), _lib.ptr(self.op._material_field),
+                  _lib.host_doubles(m), float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
+
+    def certify(self):
+        """u <- P u; r = f - K u; absolute = |r| (``resident.py:133-136``)."""
+        _lib.call('pgj_resident_certify', self.op.grid, self.batch, _lib.ptr(self.op._material_field),
+                  _lib.ptr(self.u), _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
+
+    def initialize(self):
+        """active mask and first direction z = M^-1 r (``resident.py:119-123``)."""
+        _lib.call('pgj_resident_init', self.op.grid, self.batch, _lib.ptr(self.op._plan), _lib.ptr(self.scale),
+                  _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(self.ws), _lib.stream())
+
+    def iteration(self, count=None):
+        """``count`` fused masked PCG iterations (``resident.py:125-131``)."""
+        _lib.call('pgj_resident_iterate', self.op.grid, self.batch, _lib.ptr(self.op._plan),
+                  _lib.ptr(self.op._material_field), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
+                  _lib.ptr(self.u), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
+                  _lib.ptr(self.zv), _lib.ptr(self.state), _lib.ptr(self.ws), int(count or self.chunk),
+                  _lib.stream())
+
+    def capture(self):
+        """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``)."""
+        self._setup(range(self.batch), 1e-8)
+        self.r.normal_()
+        self.certify()
+        self.initialize()
+        self.iteration()
+        torch.cuda.synchronize()
+        for name, fn in (('init_graph', self.initialize), ('iter_graph', self.iteration),
+                         ('cert_graph', self.certify)):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                fn()
+            setattr(self, name, g)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.solutions.zero_()
+        return self
+
+    # -- design-step API -------------------------------------------------------------
+    def material(self, a):
+        """Set the SIMP field a (n, n, n) (``resident.py:152-154``)."""
+        self.op._material_field.copy_(torch.as_tensor(a, dtype=torch.float64).reshape(self.shape))
+        self.op._diagonal = None
+
+    def select(self, method):
+        """alpha by name ('green', 'gj') or value; rebuilds s and its spectrum (``resident.py:156-174``)."""
+        if method == 'green':
+            alpha = 0.0
+        elif method == 'gj':
+            alpha = 1.0
+        else:
+            alpha = float(method)
+        self.alpha.fill_(alpha)
+        g = self.op.grid
+        if alpha == 0.0:
+            self.scale.fill_(1.0)
+        else:
+            ws = torch.empty(_lib.lib().pgj_scale_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=self.scale.device)
+            _lib.call('pgj_diagonal_scale', g, _lib.ptr(self.op._material_field), alpha, None, _lib.ptr(self.scale),
+                      _lib.ptr(ws), _lib.stream())
+        _lib.call('pgj_scale_spectrum', g, _lib.ptr(self.op._plan), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
+                  _lib.ptr(self.op._workspace(1)), _lib.stream())
+        return alpha
+
+    def solve(self, rtol=1e-8):
+        """Solve every load; returns (infos, host_checks) (``resident.py:176-184``, ``reproduction.py:144-151``)."""
+        infos, checks = [], 0
+        for start in range(0, len(self.loads), self.batch):
+            idx = list(range(start, start + self.batch))
+            self.u.copy_(self.solutions[start:start + self.batch])
+            info, count = self._solve_batch(idx, rtol)
+            self.solutions[start:start + self.batch].copy_(self.u)
+            infos.extend(info)
+            checks += count
+        return infos, checks
+
+    def _solve_batch(self, load_indices, rtol):
+        """One group: set-up, certify, PCG with restarts, certify (``resident.py:186-209``)."""
+        if self.iter_graph is None:
+            raise RuntimeError('call capture() first')
+        self._setup(load_indices, rtol)
+        restarts = host_checks = 0
+        self.cert_graph.replay()
+        while True:
+            self.init_graph.replay()
+            while True:
+                host_checks += 1
+                if not bool(self.active.any().item()):
+                    break
+                self.iter_graph.replay()
+            self.cert_graph.replay()
+            diag = torch.stack((self.norm, self.absolute, self.target2.sqrt(), self.iterations.to(torch.float64),
+                                self.bad.to(torch.float64)), dim=1).cpu().numpy()
+            ok = np.isfinite(diag[:, 1]) & (diag[:, 1] <= diag[:, 2]) & (diag[:, 4] == 0)
+            if ok.all() or restarts >= 2 or np.any(diag[:, 4]) or np.any((~ok) & (diag[:, 3] >= 4000)):
+                break
+            restarts += 1
+        info = [dict(rhs_norm=float(v[0]), absolute_residual=float(v[1]), threshold=float(v[2]),
+                     iterations=int(v[3]), relative_residual=float(v[1] / max(v[0], 1e-300)), converged=bool(ok[j]),
+                     restarts=restarts) for j, v in enumerate(diag)]
+        if not ok.all():
+            raise RuntimeError(('true residual certification failed', info))
+        return info, host_checks
+
+    # -- responses ---------------------------------------------------------------------
+    def energies(self, solutions=None, loads=None):
+        """Element energies (L, N) of the solutions (``energy.cu:7-35``)."""
+        sol = self.solutions if solutions is None else solutions
+        loads = self.loads if loads is None else np.atleast_2d(loads)
+        out = torch.empty((sol.shape[0], sol.shape[1] // 3), dtype=torch.float64, device=sol.device)
+        _lib.call('pgj_hex8_energy', self.op.grid, sol.shape[0], _lib.ptr(sol.contiguous()),
+                  _lib.host_doubles(np.asarray(loads, np.float64).reshape(-1)), _lib.ptr(out), _lib.stream())
+        return out
+
+    def effective_tensor(self):
+        """C^H (6, 6) from the six unit-strain solutions (``reproduction.py:166-176``)."""
+        if len(self.loads) != 6 or not np.array_equal(self.loads, LOADS6):
+            raise ValueError('C^H needs the six unit Voigt strains')
+        _lib.call('pgj_effective_tensor', self.op.grid, _lib.ptr(self.op._material_field), _lib.ptr(self.solutions),
+                  _lib.ptr(self.ch_ws), _lib.ptr(self.ch_out), _lib.stream())
+        return self.ch_out.reshape(6, 6)
+
+    def energy(self, objective='shear', iteration=0):
+        """(objective, local sensitivity density, diagonal moduli) (``reproduction.py:153-164``)."""
+        e = self.energies()
+        a = self.op._material_field.reshape(1, -1)
+        diagonal = (e * a).mean(1)
+        if objective == 'shear':
+            local = -e[3:].sum(0)
+        else:
+            hydro_u = self.solutions[:3].sum(0, keepdim=True)
+            hydro = self.energies(hydro_u, np.array([[1., 1., 1., 0., 0., 0.]]))[0]
+            if objective == 'bulk':
+                local = -hydro
+            elif objective == 'auxetic':
+                local = hydro - (1 + .8 ** iteration) * e[:3].sum(0)
+            else:
+                raise ValueError(objective)
+        obj = (local * a.reshape(-1)).mean()
+        return obj, local.reshape(self.shape), diagonal

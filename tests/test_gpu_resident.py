@@ -1,0 +1,90 @@
+"""Resident multi-load PCG on the GPU vs the oracle recurrence (GPU)."""
+import numpy as np
+import pytest
+import torch
+
+import pgj
+from oracle import design, green, hex8, pcg
+
+pytestmark = pytest.mark.gpu
+LAM, MU = 15 / 26, 5 / 13
+C0 = hex8.isotropic(LAM, MU)
+
+
+def density(n, emin, radius, seed=20261104):
+    return design.simp(design.filtered_noise(n, seed, radius), emin)
+
+
+@pytest.mark.parametrize('n,emin,alpha,mode', [(6, 1e-4, .5, 'serial'), (8, 1e-4, .5, 'parallel'),
+                                               (8, 1e-6, 0., 'parallel'), (16, 1e-4, .5, 'serial'),
+                                               (16, 1e-6, 1., 'parallel')])
+def test_resident_matches_oracle(n, emin, alpha, mode):
+    a = density(n, emin, 2 if n >= 8 else 1)
+    r = pgj.Resident(n, load_mode=mode).capture()
+    r.material(torch.tensor(a, device='cuda'))
+    r.select(str(alpha))
+    infos, checks = r.solve(rtol=1e-10)
+    s = None if alpha == 0 else green.scale(a, alpha, C0)
+    ref = pcg.ResidentPCG(a, C0, np.eye(6), batch=1 if mode == 'serial' else 3, s=s)
+    rinfo, _ = ref.solve(rtol=1e-10)
+    for gi, oi in zip(infos, rinfo):
+        assert gi['converged'] and gi['relative_residual'] <= 1e-10
+        assert abs(gi['iterations'] - oi['iterations']) <= 1, (gi, oi)
+        assert gi['restarts'] == oi['restarts']
+    u = r.solutions.cpu().numpy()
+    # the certified residual, recomputed independently on the CPU
+    for l in range(6):
+        f = hex8.rhs(a, np.eye(6)[l], C0)
+        res = np.linalg.norm(f - hex8.apply(a, u[l:l + 1], C0)) / np.linalg.norm(f)
+        assert res <= 1.0001e-10, (l, res)
+    H = r.effective_tensor().cpu().numpy()
+    Href = design.effective_tensor(a, ref.solutions, C0)
+    assert np.abs(H - Href).max() / np.abs(Href).max() < 1e-10
+
+
+def test_zero_rhs_and_warm_start():
+    n = 8
+    r = pgj.Resident(n, load_mode='parallel', batch=6).capture()
+    r.material(torch.full((n, n, n), .13, dtype=torch.float64, device='cuda'))
+    r.select('0.5')
+    infos, checks = r.solve()
+    # one group of six loads: a single host check finds no active load
+    assert all(i['iterations'] == 0 and i['converged'] for i in infos) and checks == 1
+    a = density(n, 1e-4, 2)
+    r.material(torch.tensor(a, device='cuda'))
+    r.select('0.5')
+    infos, _ = r.solve()
+    first = [i['iterations'] for i in infos]
+    infos2, _ = r.solve()   # warm start from the certified solutions
+    assert all(i['iterations'] <= f for i, f in zip(infos2, first))
+    H = r.effective_tensor().cpu().numpy()
+    assert np.abs(H - H.T).max() < 1e-12 * np.abs(H).max()
+
+
+def test_solid_effective_tensor_is_c0():
+    n = 4
+    r = pgj.Resident(n, load_mode='parallel', batch=6).capture()
+    r.material(torch.ones((n, n, n), dtype=torch.float64, device='cuda'))
+    r.select('0.5')
+    r.solve(1e-11)
+    H = r.effective_tensor().cpu().numpy()
+    assert np.abs(H - C0).max() < 1e-11
+
+
+def test_resident_64_certifies():
+    """64^3 workload (Emin 1e-6, radius-4 noise): every load certified, residual rechecked on the CPU."""
+    n = 64
+    a = density(n, 1e-6, 4)
+    r = pgj.Resident(n, load_mode='parallel', batch=6).capture()
+    r.material(torch.tensor(a, device='cuda'))
+    r.select('0.5')
+    infos, _ = r.solve()
+    assert all(i['converged'] for i in infos)
+    u = r.solutions.cpu().numpy()
+    for l in (0, 3):
+        f = hex8.rhs(a, np.eye(6)[l], C0)
+        res = np.linalg.norm(f - hex8.apply(a, u[l:l + 1], C0)) / np.linalg.norm(f)
+        assert res <= 1.0001e-8, (l, res)
+    H = r.effective_tensor().cpu().numpy()
+    assert np.abs(H - H.T).max() < 1e-10 * np.abs(H).max()
+    assert np.linalg.eigvalsh(H).min() > 0
