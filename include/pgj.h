@@ -154,6 +154,68 @@ int pgj_resident_init(const pgj_grid* g, int L, const void* plan, const double* 
 int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* s,
                          const void* s_hat, double* u, double* r, double* p0, double* p1, double* q, double* z,
                          void* state, void* ws, int count, void* stream);
+/* One stage of one iteration (0: direction update + K apply, 1: vector update +
+ * forward xy FFT, 2: z pass + Green, 3: inverse xy FFT + projection), reading
+ * the direction from pold and writing pnew; pgj_resident_iterate is these four
+ * stages with p0/p1 alternating. Exposed so each kernel can be timed alone. */
+int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, const double* a, const double* s,
+                       const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q, double* z,
+                       void* state, void* ws, int stage, void* stream);
+
+/* ------------------------------------------------------------ OC updates */
+
+/* Fields are (n, n, n) FP64. `ws` holds pgj_oc_ws_bytes(n) bytes of per-block
+ * partials (volume sums or maxima). Products are evaluated without FMA
+ * contraction, in the reference's operation order. */
+size_t pgj_oc_ws_bytes(int n);
+
+/* Clipped cone filter y_i = sum_j w_j x[i + o_j] over in-range neighbours,
+ * offsets (m, 3) int32 in (dx, dy, dz). With conditional != 0 it does nothing
+ * once bounds[2] == 0. Replaces `filter`
+ * (exp/asg_sctop_reproduction_20261105/outer.cu:8-19, :218-224). */
+int pgj_cone_filter(int n, const double* x, const int32_t* offsets, const double* weights, int m, double* y,
+                    const double* bounds, int conditional, void* stream);
+/* eta-projection of t/hs: ph, dp, a = Emin + (1-Emin) ph^3; ws (nullable)
+ * receives volume partials. Replaces `project` (outer.cu:20-28, :225-229). */
+int pgj_cone_project(int n, const double* t, const double* hs, double* ph, double* dp, double* a,
+                     const double* bounds, double beta, double eta, double emin, int conditional, double* ws,
+                     void* stream);
+/* gin = dp (3 (1-Emin) ph^2) e / hs, dvin = dp / hs (reproduction.py:92-94). */
+int pgj_cone_sensitivity(int n, const double* dp, const double* ph, const double* energy, const double* hs,
+                         double emin, double* gin, double* dvin, void* stream);
+/* g /= n^3, ratio = -g/dv, bounds = [0, 1e9, 1, 0] (reproduction.py:93-96). */
+int pgj_cone_ratio(int n, double* g, const double* dv, double* ratio, double* bounds, void* stream);
+/* OC candidate clip(x (ratio/mid)^0.3, x -+ move, [0.001, 1]) while bounds[2] != 0.
+ * Replaces `candidate` (outer.cu:29-34) with the move limit as a parameter. */
+int pgj_cone_candidate(int n, const double* x, const double* ratio, const double* bounds, double move, double* y,
+                       void* stream);
+/* One bisection step on the volume (mean of the projected candidate, from
+ * pgj_cone_project's ws) with relative stop 1e-4; bounds[3] counts steps.
+ * Replaces `bisect` (outer.cu:35-39). */
+int pgj_cone_bisect(int n, const double* ws, double* bounds, double volume, void* stream);
+/* out[0] = max |a - b| (the design change). */
+int pgj_absmax_diff(int n, const double* a, const double* b, double* ws, double* out, void* stream);
+
+/* Periodic separable Gaussian (axes z, y, x) in scipy.ndimage's symmetric
+ * accumulation order; weights has 2*radius+1 entries. Replaces `filter`
+ * (exp/asg_gpu_resident_20261103/outer.cu:7-17, :100-107). */
+int pgj_gauss_filter(int n, const double* x, const double* weights, int radius, double* tmp1, double* tmp2,
+                     double* y, void* stream);
+/* tanh projection around 0.5 and a = Emin + (1-Emin) ph^3; ws (nullable)
+ * receives volume partials. Replaces `physical` (outer.cu:18-23, :108-114). */
+int pgj_gauss_project(int n, const double* t, double beta, double* ph, double* dp, double* a, double emin,
+                      double* ws, void* stream);
+/* gin = dp p (1-Emin) ph^2 e (outer.py:35). */
+int pgj_gauss_sensitivity(int n, const double* dp, const double* ph, const double* energy, double emin,
+                          double penal, double* gin, void* stream);
+/* ratio = max(g,1e-30)/max(dv,1e-30), bounds = [0, 1e4 max ratio] (outer.py:37-38). */
+int pgj_gauss_ratio(int n, const double* g, const double* dv, double* ratio, double* ws, double* bounds,
+                    void* stream);
+/* candidate clip(x sqrt(ratio/lam), [x-move, x+move] cap [0,1]) (outer.cu:24-29). */
+int pgj_gauss_candidate(int n, const double* x, const double* ratio, const double* bounds, double move, double* y,
+                        void* stream);
+/* bisection on the mean of the projected candidate (outer.cu:30-33). */
+int pgj_gauss_bisect(int n, const double* ws, double* bounds, double volume, void* stream);
 
 #ifdef __cplusplus
 }

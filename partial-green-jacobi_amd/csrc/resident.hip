@@ -134,36 +134,46 @@ extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, con
   return launch_fxyi_mode(A, L, kFxyiPCG, s);
 }
 
-extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
-                                    const void* s_hat, double* u, double* r, double* p0, double* p1, double* q,
-                                    double* z, void* state, void* ws, int count, void* stream) {
+extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
+                                  const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q,
+                                  double* z, void* state, void* ws, int stage, void* stream) {
   if (int e = check_fft(g)) return e;
-  PGJ_REQUIRE(L >= 1 && L <= 64 && plan && a && u && r && p0 && p1 && q && z && state && ws,
-              "invalid iterate arguments");
-  PGJ_REQUIRE(count >= 2 && count % 2 == 0, "iteration count must be even and >= 2, got %d", count);
+  PGJ_REQUIRE(L >= 1 && L <= 64 && plan && a && u && r && pold && pnew && q && z && state && ws,
+              "invalid stage arguments");
+  PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   PGJ_REQUIRE(rowmarch_config(*g).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
   hipStream_t s = as_stream(stream);
   ResidentBuffers b = resident_buffers(*g, L, ws);
   StateView st = state_view(state, L);
   FFTArgs F = fft_args(*g, plan);
-  F.u = u; F.r = r; F.q = q; F.s = sc; F.shat = static_cast<const double2*>(s_hat);
+  F.u = u; F.r = r; F.q = q; F.p = pnew; F.s = sc; F.shat = static_cast<const double2*>(s_hat);
   F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z;
   F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g);
   F.st = st;
-  const int pg_count = F.f.nkb * g->ny;
-  for (int i = 0; i < count; ++i) {
-    double* pold = (i % 2 == 0) ? p0 : p1;
-    double* pnew = (i % 2 == 0) ? p1 : p0;
+  if (stage == 0) {
+    const int pg_count = F.f.nkb * g->ny;
     RowArgs K{};
     K.a = a; K.u = z; K.out = q; K.pold = pold; K.pnew = pnew; K.st = st;
     K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
     K.ppq = b.ppq; K.ppq_stride = F.ppq_count;
-    if (int e = launch_rowmarch(*g, L, kModePCG, K, s)) return e;
-    F.p = pnew;
-    if (int e = launch_fxy_mode(F, L, kFxyPCG, s)) return e;
-    if (int e = launch_fz_mode(F, L, kFzPCG, s)) return e;
-    if (int e = launch_fxyi_mode(F, L, kFxyiPCG, s)) return e;
+    return launch_rowmarch(*g, L, kModePCG, K, s);
+  }
+  if (stage == 1) return launch_fxy_mode(F, L, kFxyPCG, s);
+  if (stage == 2) return launch_fz_mode(F, L, kFzPCG, s);
+  return launch_fxyi_mode(F, L, kFxyiPCG, s);
+}
+
+extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
+                                    const void* s_hat, double* u, double* r, double* p0, double* p1, double* q,
+                                    double* z, void* state, void* ws, int count, void* stream) {
+  PGJ_REQUIRE(count >= 2 && count % 2 == 0, "iteration count must be even and >= 2, got %d", count);
+  for (int i = 0; i < count; ++i) {
+    double* pold = (i % 2 == 0) ? p0 : p1;
+    double* pnew = (i % 2 == 0) ? p1 : p0;
+    for (int stage = 0; stage < 4; ++stage)
+      if (int e = pgj_resident_stage(g, L, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws, stage, stream))
+        return e;
   }
   return PGJ_OK;
 }

@@ -59,6 +59,26 @@ SIGNATURES = {
     'pgj_resident_iterate': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      c_void_p]),
+    'pgj_resident_stage': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    'pgj_oc_ws_bytes': (c_size_t, [c_int]),
+    'pgj_cone_filter': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    'pgj_cone_project': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                 c_double, c_double, c_int, c_void_p, c_void_p]),
+    'pgj_cone_sensitivity': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p,
+                                     c_void_p]),
+    'pgj_cone_ratio': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_cone_candidate': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    'pgj_cone_bisect': (c_int, [c_int, c_void_p, c_void_p, c_double, c_void_p]),
+    'pgj_absmax_diff': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_gauss_filter': (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_gauss_project': (c_int, [c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                  c_void_p]),
+    'pgj_gauss_sensitivity': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
+                                      c_void_p]),
+    'pgj_gauss_ratio': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_gauss_candidate': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    'pgj_gauss_bisect': (c_int, [c_int, c_void_p, c_void_p, c_double, c_void_p]),
 }
 
 _lib = None
