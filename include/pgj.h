@@ -91,6 +91,10 @@ size_t pgj_scale_ws_bytes(const pgj_grid* g);
 int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alpha, double* d, double* s, double* ws,
                        void* stream);
 
+/* Layout conversions between AoS (L, N, 3) and SoA (L, 3, N) vectors. */
+int pgj_aos_to_soa(const pgj_grid* g, int L, const double* in, double* out, void* stream);
+int pgj_soa_to_aos(const pgj_grid* g, int L, const double* in, double* out, void* stream);
+
 /* Zero-mean projection per load and component, in place.
  * Replaces `project_zero_mean` (solver/src/materials/hybrid_adapter.py:78-80). */
 size_t pgj_project_ws_bytes(const pgj_grid* g, int L);
@@ -125,6 +129,11 @@ int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, const double* 
 
 /* ------------------------------------------------------ resident PCG (FP64) */
 
+/* The resident solver keeps its vectors (u, r, p, q, z, rhs) and the scale s
+ * in SoA layout, (L, 3, N) and (3, N): one contiguous plane per component, so
+ * the FFT passes stream whole component planes. Convert with pgj_aos_to_soa /
+ * pgj_soa_to_aos at the boundary. */
+
 /* Per-load solver state, `state` of pgj_resident_state_bytes(L) bytes:
  *   double rz[L], target2[L], norm[L], absolute[L];
  *   int64  iterations[L];
@@ -135,6 +144,11 @@ size_t pgj_resident_state_bytes(int L);
 int pgj_resident_state_offsets(int L, size_t* offsets7);
 size_t pgj_resident_ws_bytes(const pgj_grid* g, int L);
 
+/* Partial Green-Jacobi scale s (SoA, (3, N)) of the field a for exponent
+ * alpha (alpha = 0 gives s = 1) and its spectrum s_hat (3, nz, ny, nx/2+1)
+ * complex128. Replaces Resident.select (resident.py:156-174). */
+int pgj_resident_scale(const pgj_grid* g, const void* plan, const double* a, double alpha, double* s, void* s_hat,
+                       void* ws, void* stream);
 /* rhs_l for the L macro strains (host L*6), norms, targets
  * max(rtol |f|, 1e-13/sqrt(nx))^2, iterations = 0, bad = 0.
  * Replaces Resident._solve_batch set-up (resident.py:186-191). */

@@ -11,62 +11,11 @@
 #include <cmath>
 
 #include "fft_kernels.h"
+#include "stockham.h"
 
 namespace pgj {
 
 constexpr int kMaxV = kStageValues;
-
-__device__ __forceinline__ double2 c2(double x, double y) { return make_double2(x, y); }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return c2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return c2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return c2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ double2 conjd(double2 a) { return c2(a.x, -a.y); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return c2(a.x * s, a.y * s); }
-
-// W_n^k from a table of length T = n * step (W_T[j] = exp(-2 pi i j / T)).
-template <bool INV>
-__device__ __forceinline__ double2 twiddle(const double2* tw, int k, int step) {
-  const double2 t = tw[k * step];
-  return INV ? conjd(t) : t;
-}
-
-template <int R, bool INV>
-__device__ __forceinline__ void dft(double2 (&v)[R], const double2* tw, int n, int step) {
-  if constexpr (R == 2) {
-    const double2 a = v[0];
-    v[0] = cadd(a, v[1]);
-    v[1] = csub(a, v[1]);
-  } else if constexpr (R == 4) {
-    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]);
-    double2 t3 = csub(v[1], v[3]);
-    t3 = INV ? c2(-t3.y, t3.x) : c2(t3.y, -t3.x);
-    v[0] = cadd(t0, t2);
-    v[2] = csub(t0, t2);
-    v[1] = cadd(t1, t3);
-    v[3] = csub(t1, t3);
-  } else if constexpr (R == 3) {
-    constexpr double h = 0.86602540378443864676;  // sqrt(3)/2
-    const double2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-    const double2 m = c2(v[0].x - 0.5 * s.x, v[0].y - 0.5 * s.y);
-    const double2 jd = INV ? c2(-h * d.y, h * d.x) : c2(h * d.y, -h * d.x);
-    v[0] = cadd(v[0], s);
-    v[1] = cadd(m, jd);
-    v[2] = csub(m, jd);
-  } else {
-    double2 out[R];
-#pragma unroll
-    for (int p = 0; p < R; ++p) {
-      double2 acc = v[0];
-#pragma unroll
-      for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(v[q], twiddle<INV>(tw, ((p * q) % R) * (n / R), step)));
-      out[p] = acc;
-    }
-#pragma unroll
-    for (int p = 0; p < R; ++p) v[p] = out[p];
-  }
-}
 
 // Placement of the lines of a batched transform in LDS: line l starts at
 // (l / per) * outer + (l % per) * inner and its elements are es apart.
@@ -166,7 +115,6 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
   const int l = blockIdx.z;
   const int nx = A.nx, ny = A.ny;
   const int64_t plane_nodes = int64_t(nx) * ny;
-  const int64_t vbase = int64_t(l) * 3 * A.N + int64_t(z) * 3 * plane_nodes;
   double* red = reinterpret_cast<double*>(plane + size_t(comps) * ny * f.rs);
 
   double alpha = 0.0;
@@ -190,15 +138,17 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
   for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
     const int64_t node = comps == 3 ? e / 3 : e;
     const int c = comps == 3 ? int(e % 3) : c0;
-    const int64_t idx = vbase + 3 * node + c;
+    const int64_t gnode = int64_t(z) * plane_nodes + node;
+    const int64_t idx = vindex(A, l, c, gnode);
+    const int64_t sidx = vindex(A, 0, c, gnode);
     double val;
     if (MODE == kFxyScale) {
-      val = A.s[int64_t(z) * 3 * plane_nodes + 3 * node + c];
+      val = A.s[sidx];
     } else if (MODE == kFxyPre) {
       const double rv = A.rin[idx];
       sq[c] = fma(rv, rv, sq[c]);
       sm[c] += rv;
-      val = A.s ? A.s[int64_t(z) * 3 * plane_nodes + 3 * node + c] * rv : rv;
+      val = A.s ? A.s[sidx] * rv : rv;
     } else {
       const double un = fma(alpha, A.p[idx], A.u[idx]);
       const double rn = fma(-alpha, A.q[idx], A.r[idx]);
@@ -206,7 +156,7 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
       A.r[idx] = rn;
       sq[c] = fma(rn, rn, sq[c]);
       sm[c] += rn;
-      val = A.s ? A.s[int64_t(z) * 3 * plane_nodes + 3 * node + c] * rn : rn;
+      val = A.s ? A.s[sidx] * rn : rn;
     }
     const int x = int(node % nx), y = int(node / nx);
     double2* row = plane + (size_t(comps == 3 ? c : 0) * ny + y) * f.rs;
@@ -447,7 +397,6 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
   // 4. inverse x transform
   lds_fft<true>(plane, rows_map(rows, f.rs), f.fx, A.twx, nx / f.M);
   // 5. unpack, scale and project
-  const int64_t vbase = int64_t(l) * 3 * A.N + int64_t(z) * 3 * plane_nodes;
   const int64_t count = comps == 3 ? 3 * plane_nodes : plane_nodes;
   for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
     const int64_t node = comps == 3 ? e / 3 : e;
@@ -455,8 +404,9 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
     const int x = int(node % nx), y = int(node / nx);
     const double2* row = plane + (size_t(comps == 3 ? c : 0) * ny + y) * f.rs;
     const double w = even ? reinterpret_cast<const double*>(row)[x] : row[x].x;
-    const double v = A.s ? A.s[int64_t(z) * 3 * plane_nodes + 3 * node + c] * w - mean[c] : w;
-    A.zout[vbase + 3 * node + c] = v;
+    const int64_t gnode = int64_t(z) * plane_nodes + node;
+    const double v = A.s ? A.s[vindex(A, 0, c, gnode)] * w - mean[c] : w;
+    A.zout[vindex(A, l, c, gnode)] = v;
   }
 }
 

@@ -33,12 +33,23 @@ struct FFTArgs {
   const double* ppq;     // hex8 partials [l][blk]
   int ppq_count;
   StateView st;
+  int soa;               // vectors (and s) are SoA (L, 3, N) instead of AoS (L, N, 3)
 };
+
+// Index of component c of node `node` of load l in a vector (and of s with l = 0).
+__device__ __forceinline__ int64_t vindex(const FFTArgs& A, int l, int c, int64_t node) {
+  return A.soa ? (int64_t(l) * 3 + c) * A.N + node : (int64_t(l) * A.N + node) * 3 + c;
+}
 
 FFTArgs fft_args(const pgj_grid& g, const void* plan);
 int check_fft(const pgj_grid* g);
 int launch_fxy_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s);
+// Compile-time-shaped SoA kernels for cubic grids (n in 16, 24, 32, 48, 64, 96, 128).
+bool static_fft(const pgj_grid& g);
+int launch_fxy_static(FFTArgs& A, int L, int mode, hipStream_t s);
+int launch_fz_static(FFTArgs& A, int L, int mode, hipStream_t s);
+int launch_fxyi_static(FFTArgs& A, int L, hipStream_t s);
 
 }  // namespace pgj

@@ -17,6 +17,8 @@ __device__ __forceinline__ double node_value(const RowArgs& A, int64_t idx, doub
 
 template <int MODE, int XPL, int R, bool CUBIC>
 __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
+  // the drop-in apply works on AoS vectors, the resident iteration on SoA
+  constexpr bool SOA = MODE == kModePCG;
   extern __shared__ double lds[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l = blockIdx.z;
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
         const int64_t node = (int64_t(plane) * ny + y) * nx + xl * XPL + xi;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const int64_t idx = base + 3 * node + d;
+          const int64_t idx = SOA ? (int64_t(l) * 3 + d) * A.N + node : base + 3 * node + d;
           const double v = node_value<MODE>(A, idx, beta, first);
           dst[j][xi][d] = v;
           if (MODE == kModePCG && write_p && j < rows && !halo && lane_on) A.pnew[idx] = v;
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
             const int64_t node = (int64_t(zl) * ny + (ybase + j)) * nx + xl * XPL + xi;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-              A.out[base + 3 * node + d] = accL[j][xi][d];
+              A.out[SOA ? (int64_t(l) * 3 + d) * A.N + node : base + 3 * node + d] = accL[j][xi][d];
               if (MODE == kModePCG) pq = fma(nodeL[j][xi][d], accL[j][xi][d], pq);
             }
           }
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
                   nzp = wrapi(ez + ((n >> 2) & 1), A.nz);
         const int64_t node = (int64_t(nzp) * A.ny + nyp) * A.nx + nxp;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ue[3 * n + d] = A.u[base + 3 * node + d];
+        for (int d = 0; d < 3; ++d) ue[3 * n + d] = A.u[A.soa ? (int64_t(l) * 3 + d) * A.N + node : base + 3 * node + d];
       }
       const double ae = A.a[(int64_t(ez) * A.ny + ey) * A.nx + ex];
       hex8_element<CUBIC>(ue, ae, A.coef, fe);
@@ -246,12 +248,13 @@ __global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
       for (int d = 0; d < 3; ++d) acc[d] += fe[3 * o + d];
     }
     for (int d = 0; d < 3; ++d) {
+      const int64_t idx = A.soa ? (int64_t(l) * 3 + d) * A.N + i : base + 3 * i + d;
       if (CERT) {
-        const double r = A.rhs[base + 3 * i + d] - acc[d];
-        A.out[base + 3 * i + d] = r;
+        const double r = A.rhs[idx] - acc[d];
+        A.out[idx] = r;
         res2 = fma(r, r, res2);
       } else {
-        A.out[base + 3 * i + d] = acc[d];
+        A.out[idx] = acc[d];
       }
     }
   }
@@ -262,39 +265,57 @@ __global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
 }
 
 // Per-block component sums of a (L, 3N) vector into partial rows of 4 doubles.
-__global__ __launch_bounds__(256) void component_sums(const double* v, int64_t N, double* part) {
+__device__ __forceinline__ int64_t lidx(int soa, int l, int d, int64_t i, int64_t N) {
+  return soa ? (int64_t(l) * 3 + d) * N + i : (int64_t(l) * N + i) * 3 + d;
+}
+
+__global__ __launch_bounds__(256) void component_sums(const double* v, int64_t N, double* part, int soa) {
   __shared__ double red[8];
   const int l = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   double s[3] = {0.0, 0.0, 0.0};
   if (i < N)
-    for (int d = 0; d < 3; ++d) s[d] = v[int64_t(l) * 3 * N + 3 * i + d];
+    for (int d = 0; d < 3; ++d) s[d] = v[lidx(soa, l, d, i, N)];
   for (int d = 0; d < 3; ++d) {
     const double t = block_sum(s[d], red);
     if (threadIdx.x == 0) part[(int64_t(l) * gridDim.x + blockIdx.x) * 4 + d] = t;
   }
 }
 
-// v -= mean per load/component (mean from component_sums partials).
-__global__ __launch_bounds__(256) void subtract_mean(double* v, int64_t N, const double* part, int nblk) {
+// v -= mean per load/component (mean from component_sums partials; the first
+// wave reduces them in a fixed order and shares the result through LDS).
+__global__ __launch_bounds__(256) void subtract_mean(double* v, int64_t N, const double* part, int nblk, int soa) {
+  __shared__ double mean[3];
   const int l = blockIdx.y;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const double* p = part + int64_t(l) * nblk * 4;
-  double mean[3];
-  for (int d = 0; d < 3; ++d) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += p[b * 4 + d];
-    mean[d] = s / double(N);
+  if (threadIdx.x < 64) {
+    const double* p = part + int64_t(l) * nblk * 4;
+    for (int d = 0; d < 3; ++d) {
+      const double s = wave_reduce_partials(p + d, nblk, 4);
+      if (threadIdx.x == 0) mean[d] = s / double(N);
+    }
   }
+  __syncthreads();
   if (i < N)
-    for (int d = 0; d < 3; ++d) v[int64_t(l) * 3 * N + 3 * i + d] -= mean[d];
+    for (int d = 0; d < 3; ++d) v[lidx(soa, l, d, i, N)] -= mean[d];
+}
+
+// AoS (L, N, 3) <-> SoA (L, 3, N)
+__global__ __launch_bounds__(256) void transpose_kernel(const double* in, double* out, int64_t N, int to_soa) {
+  const int l = blockIdx.y;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int d = 0; d < 3; ++d) {
+    if (to_soa) out[lidx(1, l, d, i, N)] = in[lidx(0, l, d, i, N)];
+    else out[lidx(0, l, d, i, N)] = in[lidx(1, l, d, i, N)];
+  }
 }
 
 // ------------------------------------------------------------------------ rhs
 // f_node = -sum_{8 elements} a_e V Bbar_a^T C0 eps with centroid gradients +-1/(4h).
 __global__ __launch_bounds__(256) void hex8_rhs_kernel(int nz, int ny, int nx, ElementCoef c, const double* a,
                                                        double m0, double m1, double m2, double m3, double m4,
-                                                       double m5, double* out) {
+                                                       double m5, double* out, int soa) {
   const int64_t N = int64_t(nx) * ny * nz;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -314,7 +335,7 @@ __global__ __launch_bounds__(256) void hex8_rhs_kernel(int nz, int ny, int nx, E
     f[1] -= ae * (dx * s5 + dy * s1 + dz * s3);
     f[2] -= ae * (dx * s4 + dy * s3 + dz * s2);
   }
-  for (int d = 0; d < 3; ++d) out[3 * i + d] = f[d];
+  for (int d = 0; d < 3; ++d) out[lidx(soa, 0, d, i, N)] = f[d];
 }
 
 // Affine element displacement chi_n,d = (E * (offset_n * h))_d for a macro strain.
@@ -419,10 +440,9 @@ __global__ __launch_bounds__(256) void effective_partials(int nz, int ny, int nx
 }
 
 __global__ void effective_finish(const double* part, int nblk, int64_t N, double* out) {
-  const int idx = threadIdx.x;
-  if (idx >= 21) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[int64_t(b) * 21 + idx];
+  const int idx = blockIdx.x;  // one wavefront per upper-triangle entry
+  const double s = wave_reduce_partials(part + idx, nblk, 21);
+  if (threadIdx.x != 0) return;
   int k = 0;
   for (int i = 0; i < 6; ++i)
     for (int j = i; j < 6; ++j, ++k)
@@ -466,11 +486,9 @@ __global__ __launch_bounds__(256) void avg_stress_partials(int nz, int ny, int n
 }
 
 __global__ void avg_stress_finish(const double* part, int nblk, int64_t N, double* out) {
-  const int k = threadIdx.x;
-  if (k >= 6) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[int64_t(b) * 6 + k];
-  out[k] = s / double(N);
+  const int k = blockIdx.x;
+  const double s = wave_reduce_partials(part + k, nblk, 6);
+  if (threadIdx.x == 0) out[k] = s / double(N);
 }
 
 // Nodal diagonal: d_i = sum_o a_{i-o} diag(Ke)[o]; log partials for the scale.
@@ -498,12 +516,17 @@ __global__ __launch_bounds__(256) void diagonal_kernel(int nz, int ny, int nx, c
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(int nz, int ny, int nx, const double* a, DiagTable ed,
-                                                    double alpha, const double* logpart, int nblk, double* s) {
+                                                    double alpha, const double* logpart, int nblk, double* s,
+                                                    int soa) {
+  __shared__ double smean;
   const int64_t N = int64_t(nx) * ny * nz;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  double tot = 0.0;
-  for (int b = 0; b < nblk; ++b) tot += logpart[b];
-  const double mean = tot / double(3 * N);
+  if (threadIdx.x < 64) {
+    const double tot = wave_reduce_partials(logpart, nblk, 1);
+    if (threadIdx.x == 0) smean = tot / double(3 * N);
+  }
+  __syncthreads();
+  const double mean = smean;
   if (i >= N) return;
   const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
   double acc[3] = {0.0, 0.0, 0.0};
@@ -512,7 +535,7 @@ __global__ __launch_bounds__(256) void scale_kernel(int nz, int ny, int nx, cons
     const double ae = a[(int64_t(ez) * ny + ey) * nx + ex];
     for (int k = 0; k < 3; ++k) acc[k] += ae * ed.v[o][k];
   }
-  for (int k = 0; k < 3; ++k) s[3 * i + k] = exp(-0.5 * alpha * (log(acc[k]) - mean));
+  for (int k = 0; k < 3; ++k) s[lidx(soa, 0, k, i, N)] = exp(-0.5 * alpha * (log(acc[k]) - mean));
 }
 
 
@@ -587,18 +610,18 @@ int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_
   return PGJ_OK;
 }
 
-int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, hipStream_t s) {
+int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, int soa, hipStream_t s) {
   const int64_t N = int64_t(g.nx) * g.ny * g.nz;
   const int nblk = int((N + 255) / 256);
-  hipLaunchKernelGGL(subtract_mean, dim3(nblk, L), dim3(256), 0, s, v, N, part, nblk);
+  hipLaunchKernelGGL(subtract_mean, dim3(nblk, L), dim3(256), 0, s, v, N, part, nblk, soa);
   PGJ_LAUNCHED("subtract_mean");
   return PGJ_OK;
 }
 
-int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, hipStream_t s) {
+int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, int soa, hipStream_t s) {
   const int64_t N = int64_t(g.nx) * g.ny * g.nz;
   dim3 grid(unsigned((N + 255) / 256), L);
-  hipLaunchKernelGGL(component_sums, grid, dim3(256), 0, s, v, N, part);
+  hipLaunchKernelGGL(component_sums, grid, dim3(256), 0, s, v, N, part, soa);
   PGJ_LAUNCHED("component_sums");
   return PGJ_OK;
 }
@@ -634,10 +657,34 @@ extern "C" int pgj_hex8_rhs(const pgj_grid* g, const double* a, const double mac
   PGJ_REQUIRE(a && macro && out, "invalid rhs arguments");
   const int64_t N = int64_t(g->nx) * g->ny * g->nz;
   hipLaunchKernelGGL(hex8_rhs_kernel, dim3(unsigned((N + 255) / 256)), dim3(256), 0, as_stream(stream), g->nz, g->ny,
-                     g->nx, coef_of(*g), a, macro[0], macro[1], macro[2], macro[3], macro[4], macro[5], out);
+                     g->nx, coef_of(*g), a, macro[0], macro[1], macro[2], macro[3], macro[4], macro[5], out, 0);
   PGJ_LAUNCHED("hex8_rhs");
   return PGJ_OK;
 }
+
+namespace pgj {
+
+int launch_rhs(const pgj_grid& g, const double* a, const double* macro, double* out, int soa, hipStream_t s) {
+  const int64_t N = int64_t(g.nx) * g.ny * g.nz;
+  hipLaunchKernelGGL(hex8_rhs_kernel, dim3(unsigned((N + 255) / 256)), dim3(256), 0, s, g.nz, g.ny, g.nx, coef_of(g),
+                     a, macro[0], macro[1], macro[2], macro[3], macro[4], macro[5], out, soa);
+  PGJ_LAUNCHED("hex8_rhs");
+  return PGJ_OK;
+}
+
+int launch_scale(const pgj_grid& g, const double* a, double alpha, double* d, double* s_out, double* ws, int soa,
+                 hipStream_t s) {
+  const int64_t N = int64_t(g.nx) * g.ny * g.nz;
+  const int nblk = int((N + 255) / 256);
+  DiagTable t = diag_table(g);
+  hipLaunchKernelGGL(diagonal_kernel, dim3(nblk), dim3(256), 0, s, g.nz, g.ny, g.nx, a, t, d, ws);
+  PGJ_LAUNCHED("diagonal_kernel");
+  hipLaunchKernelGGL(scale_kernel, dim3(nblk), dim3(256), 0, s, g.nz, g.ny, g.nx, a, t, alpha, ws, nblk, s_out, soa);
+  PGJ_LAUNCHED("scale_kernel");
+  return PGJ_OK;
+}
+
+}  // namespace pgj
 
 extern "C" int pgj_hex8_energy(const pgj_grid* g, int L, const double* u, const double* macros, double* out,
                                void* stream) {
@@ -676,7 +723,7 @@ extern "C" int pgj_effective_tensor(const pgj_grid* g, const double* a, const do
     hipLaunchKernelGGL(effective_partials<false>, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a, u6,
                        ws);
   PGJ_LAUNCHED("effective_partials");
-  hipLaunchKernelGGL(effective_finish, dim3(1), dim3(64), 0, s, ws, nblk, N, out36);
+  hipLaunchKernelGGL(effective_finish, dim3(21), dim3(64), 0, s, ws, nblk, N, out36);
   PGJ_LAUNCHED("effective_finish");
   return PGJ_OK;
 }
@@ -697,7 +744,7 @@ extern "C" int pgj_hex8_avg_stress(const pgj_grid* g, const double* a, const dou
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(avg_stress_partials, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a, u, ms, ws);
   PGJ_LAUNCHED("avg_stress_partials");
-  hipLaunchKernelGGL(avg_stress_finish, dim3(1), dim3(64), 0, s, ws, nblk, N, out6);
+  hipLaunchKernelGGL(avg_stress_finish, dim3(6), dim3(64), 0, s, ws, nblk, N, out6);
   PGJ_LAUNCHED("avg_stress_finish");
   return PGJ_OK;
 }
@@ -711,15 +758,7 @@ extern "C" int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alp
                                   double* ws, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(a && s_out && ws, "invalid scale arguments");
-  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
-  const int nblk = int((N + 255) / 256);
-  DiagTable t = diag_table(*g);
-  hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(diagonal_kernel, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, a, t, d, ws);
-  PGJ_LAUNCHED("diagonal_kernel");
-  hipLaunchKernelGGL(scale_kernel, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, a, t, alpha, ws, nblk, s_out);
-  PGJ_LAUNCHED("scale_kernel");
-  return PGJ_OK;
+  return launch_scale(*g, a, alpha, d, s_out, ws, 0, as_stream(stream));
 }
 
 extern "C" size_t pgj_project_ws_bytes(const pgj_grid* g, int L) {
@@ -734,6 +773,24 @@ extern "C" int pgj_project(const pgj_grid* g, int L, double* v, double* ws, void
   const int nblk = int((N + 255) / 256);
   hipStream_t s = as_stream(stream);
   (void)nblk;
-  if (int e = launch_component_sums(*g, L, v, ws, s)) return e;
-  return launch_subtract_mean(*g, L, v, ws, s);
+  if (int e = launch_component_sums(*g, L, v, ws, 0, s)) return e;
+  return launch_subtract_mean(*g, L, v, ws, 0, s);
+}
+
+extern "C" int pgj_aos_to_soa(const pgj_grid* g, int L, const double* in, double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && in && out && in != out, "invalid transpose arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  hipLaunchKernelGGL(transpose_kernel, dim3(unsigned((N + 255) / 256), L), dim3(256), 0, as_stream(stream), in, out, N, 1);
+  PGJ_LAUNCHED("transpose");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_soa_to_aos(const pgj_grid* g, int L, const double* in, double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && in && out && in != out, "invalid transpose arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  hipLaunchKernelGGL(transpose_kernel, dim3(unsigned((N + 255) / 256), L), dim3(256), 0, as_stream(stream), in, out, N, 0);
+  PGJ_LAUNCHED("transpose");
+  return PGJ_OK;
 }

@@ -42,6 +42,7 @@ struct RefArgs {
   // 4-wide partial rows [l][block][4]
   const double* rhs;
   double* pres;
+  int soa;               // vectors are SoA (L, 3, N) instead of AoS (L, N, 3)
 };
 
 
@@ -50,7 +51,10 @@ ElementCoef coef_of(const pgj_grid& g);
 int check_grid(const pgj_grid* g);
 int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgs& A, hipStream_t s);
 int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_t s);
-int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, hipStream_t s);
-int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, hipStream_t s);
+int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, int soa, hipStream_t s);
+int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, int soa, hipStream_t s);
+int launch_rhs(const pgj_grid& g, const double* a, const double* macro, double* out, int soa, hipStream_t s);
+int launch_scale(const pgj_grid& g, const double* a, double alpha, double* d, double* s_out, double* ws, int soa,
+                 hipStream_t s);
 
 }  // namespace pgj

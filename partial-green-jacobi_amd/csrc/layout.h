@@ -123,6 +123,19 @@ inline FFTGeom fft_geom(const pgj_grid& g) {
   const size_t col = size_t(g.nz) * 3 * 16;
   int kc = 1;
   while (kc < f.nkx && size_t(kc + 1) * col + 64 * 8 <= 56 * 1024) ++kc;
+  if (g.nx == g.ny && g.ny == g.nz) {
+    // same kx chunking as the compile-time kernels (fft_static.hip, Cube<N>::KC)
+    switch (g.nx) {
+      case 16: kc = 9; break;
+      case 24: kc = 13; break;
+      case 32: kc = 17; break;
+      case 48: kc = 5; break;
+      case 64: kc = 11; break;
+      case 96: kc = 7; break;
+      case 128: kc = 5; break;
+      default: break;
+    }
+  }
   f.kc = kc;
   f.nkb = (f.nkx + kc - 1) / kc;
   f.lds_z = size_t(kc) * col + 64 * 8;

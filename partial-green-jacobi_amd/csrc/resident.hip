@@ -90,7 +90,7 @@ extern "C" int pgj_resident_setup(const pgj_grid* g, int L, const double* a, con
   const int64_t N = int64_t(g->nx) * g->ny * g->nz;
   hipStream_t s = as_stream(stream);
   for (int l = 0; l < L; ++l)
-    if (int e = pgj_hex8_rhs(g, a, macros + 6 * l, rhs + int64_t(l) * 3 * N, stream)) return e;
+    if (int e = launch_rhs(*g, a, macros + 6 * l, rhs + int64_t(l) * 3 * N, 1, s)) return e;
   ResidentBuffers b = resident_buffers(*g, L, ws);
   const int nblk = b.lay.nblk_cert;
   hipLaunchKernelGGL(square_partials, dim3(nblk, L), dim3(256), 0, s, rhs, N, b.pcert);
@@ -107,14 +107,40 @@ extern "C" int pgj_resident_certify(const pgj_grid* g, int L, const double* a, d
   PGJ_REQUIRE(L >= 1 && L <= 64 && a && u && rhs && r && state && ws, "invalid certify arguments");
   hipStream_t s = as_stream(stream);
   ResidentBuffers b = resident_buffers(*g, L, ws);
-  if (int e = launch_component_sums(*g, L, u, b.pcert, s)) return e;
-  if (int e = launch_subtract_mean(*g, L, u, b.pcert, s)) return e;
+  if (int e = launch_component_sums(*g, L, u, b.pcert, 1, s)) return e;
+  if (int e = launch_subtract_mean(*g, L, u, b.pcert, 1, s)) return e;
   RefArgs A{};
-  A.a = a; A.u = u; A.out = r; A.rhs = rhs; A.pres = b.pcert;
+  A.a = a; A.u = u; A.out = r; A.rhs = rhs; A.pres = b.pcert; A.soa = 1;
   if (int e = launch_reference(*g, L, A, true, s)) return e;
   hipLaunchKernelGGL(certify_finish, dim3(L), dim3(64), 0, s, b.pcert, b.lay.nblk_cert, state_view(state, L));
   PGJ_LAUNCHED("certify_finish");
   return PGJ_OK;
+}
+
+namespace pgj {
+int fxy(FFTArgs& A, const pgj_grid& g, int L, int mode, hipStream_t s) {
+  return static_fft(g) ? launch_fxy_static(A, L, mode, s) : launch_fxy_mode(A, L, mode, s);
+}
+int fz(FFTArgs& A, const pgj_grid& g, int L, int mode, hipStream_t s) {
+  return static_fft(g) ? launch_fz_static(A, L, mode, s) : launch_fz_mode(A, L, mode, s);
+}
+int fxyi(FFTArgs& A, const pgj_grid& g, int L, hipStream_t s) {
+  return static_fft(g) ? launch_fxyi_static(A, L, s) : launch_fxyi_mode(A, L, kFxyiPCG, s);
+}
+}  // namespace pgj
+
+extern "C" int pgj_resident_scale(const pgj_grid* g, const void* plan, const double* a, double alpha, double* sc,
+                                  void* s_hat, void* ws, void* stream) {
+  if (int e = check_fft(g)) return e;
+  PGJ_REQUIRE(plan && a && sc && s_hat && ws, "invalid resident scale arguments");
+  PGJ_REQUIRE(alpha >= 0.0 && alpha <= 1.0, "alpha must be in [0, 1], got %g", alpha);
+  hipStream_t s = as_stream(stream);
+  ResidentBuffers b = resident_buffers(*g, 1, ws);
+  if (int e = launch_scale(*g, a, alpha, nullptr, sc, b.pcert, 1, s)) return e;
+  FFTArgs A = fft_args(*g, plan);
+  A.s = sc; A.spec = static_cast<double2*>(s_hat); A.pr = b.pr; A.pg = b.pg; A.soa = 1;
+  if (int e = fxy(A, *g, 1, kFxyScale, s)) return e;
+  return fz(A, *g, 1, kFzScale, s);
 }
 
 extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, const double* sc, const void* s_hat,
@@ -129,9 +155,10 @@ extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, con
   A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z;
   A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g);
   A.st = state_view(state, L);
-  if (int e = launch_fxy_mode(A, L, kFxyPre, s)) return e;
-  if (int e = launch_fz_mode(A, L, kFzInit, s)) return e;
-  return launch_fxyi_mode(A, L, kFxyiPCG, s);
+  A.soa = 1;
+  if (int e = fxy(A, *g, L, kFxyPre, s)) return e;
+  if (int e = fz(A, *g, L, kFzInit, s)) return e;
+  return fxyi(A, *g, L, s);
 }
 
 extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
@@ -151,6 +178,7 @@ extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, co
   F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z;
   F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g);
   F.st = st;
+  F.soa = 1;
   if (stage == 0) {
     const int pg_count = F.f.nkb * g->ny;
     RowArgs K{};
@@ -159,9 +187,9 @@ extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, co
     K.ppq = b.ppq; K.ppq_stride = F.ppq_count;
     return launch_rowmarch(*g, L, kModePCG, K, s);
   }
-  if (stage == 1) return launch_fxy_mode(F, L, kFxyPCG, s);
-  if (stage == 2) return launch_fz_mode(F, L, kFzPCG, s);
-  return launch_fxyi_mode(F, L, kFxyiPCG, s);
+  if (stage == 1) return fxy(F, *g, L, kFxyPCG, s);
+  if (stage == 2) return fz(F, *g, L, kFzPCG, s);
+  return fxyi(F, *g, L, s);
 }
 
 extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,

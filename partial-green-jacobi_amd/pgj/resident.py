@@ -48,6 +48,7 @@ class Resident:
         B = self.batch
         dev = torch.device(device)
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        # solver vectors are SoA (B, 3, N) on the device; solutions are AoS (L, 3N)
         self.u, self.r, self.p0, self.p1, self.q, self.zv, self.rhs = (z(B, 3 * N) for _ in range(7))
         self.solutions = z(nl, 3 * N)
         lib = _lib.lib()
@@ -63,8 +64,10 @@ class Resident:
         self.active = self.state[o[5]:o[5] + B]
         self.bad = self.state[o[6]:o[6] + B]
         self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, B), dtype=torch.uint8, device=dev)
-        self.scale = torch.ones((1, 3 * N), dtype=torch.float64, device=dev)
-        self.scale_hat = self.op.scale_spectrum(self.scale)
+        self.scale = torch.ones((1, 3 * N), dtype=torch.float64, device=dev)   # SoA (3, N)
+        nz, ny, nx = shape
+        self.scale_hat = torch.zeros((3, nz * ny * (nx // 2 + 1)), dtype=torch.complex128, device=dev)
+        self._set_scale(0.0)
         self.alpha = torch.zeros((), dtype=torch.float64, device=dev)
         self.energy_out = z(nl, N)
         self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
@@ -72,8 +75,18 @@ class Resident:
         self.init_graph = self.iter_graph = self.cert_graph = None
 
     # -- kernels ---------------------------------------------------------------------
+    def to_soa(self, v):
+        out = torch.empty_like(v)
+        _lib.call('pgj_aos_to_soa', self.op.grid, v.shape[0], _lib.ptr(v), _lib.ptr(out), _lib.stream())
+        return out
+
+    def to_aos(self, v):
+        out = torch.empty_like(v)
+        _lib.call('pgj_soa_to_aos', self.op.grid, v.shape[0], _lib.ptr(v), _lib.ptr(out), _lib.stream())
+        return out
+
     def K(self, u, active=None):
-        """Scalar-a operator for (B, 3N) vectors (``resident.py:98``)."""
+        """Scalar-a operator for AoS (B, 3N) vectors (``resident.py:98``)."""
         return self.op.apply_batched(u, active)
 
     def true_K(self, u):
@@ -81,8 +94,14 @@ class Resident:
         return self.op.apply_reference(u)
 
     def B(self, r):
-        """P s G s P r (``resident.py:104-117``)."""
-        return self.op.green_precond(r, self.scale, self.scale_hat)
+        """P s G s P r for AoS (B, 3N) residuals (``resident.py:104-117``)."""
+        s = self.to_aos(self.scale)
+        return self.op.green_precond(r, s, self.scale_hat)
+
+    def _set_scale(self, alpha):
+        ws = self.op._workspace(1)
+        _lib.call('pgj_resident_scale', self.op.grid, _lib.ptr(self.op._plan), _lib.ptr(self.op._material_field),
+                  float(alpha), _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.ws), _lib.stream())
 
     def _setup(self, load_indices, rtol):
         m = self.loads[list(load_indices)].reshape(-1)
@@ -144,15 +163,7 @@ class Resident:
         else:
             alpha = float(method)
         self.alpha.fill_(alpha)
-        g = self.op.grid
-        if alpha == 0.0:
-            self.scale.fill_(1.0)
-        else:
-            ws = torch.empty(_lib.lib().pgj_scale_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=self.scale.device)
-            _lib.call('pgj_diagonal_scale', g, _lib.ptr(self.op._material_field), alpha, None, _lib.ptr(self.scale),
-                      _lib.ptr(ws), _lib.stream())
-        _lib.call('pgj_scale_spectrum', g, _lib.ptr(self.op._plan), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
-                  _lib.ptr(self.op._workspace(1)), _lib.stream())
+        self._set_scale(alpha)
         return alpha
 
     def solve(self, rtol=1e-8):
@@ -160,9 +171,11 @@ class Resident:
         infos, checks = [], 0
         for start in range(0, len(self.loads), self.batch):
             idx = list(range(start, start + self.batch))
-            self.u.copy_(self.solutions[start:start + self.batch])
+            _lib.call('pgj_aos_to_soa', self.op.grid, self.batch, _lib.ptr(self.solutions[start:start + self.batch]),
+                      _lib.ptr(self.u), _lib.stream())
             info, count = self._solve_batch(idx, rtol)
-            self.solutions[start:start + self.batch].copy_(self.u)
+            _lib.call('pgj_soa_to_aos', self.op.grid, self.batch, _lib.ptr(self.u),
+                      _lib.ptr(self.solutions[start:start + self.batch]), _lib.stream())
             infos.extend(info)
             checks += count
         return infos, checks

@@ -17,7 +17,8 @@ def density(n, emin, radius, seed=20261104):
 
 @pytest.mark.parametrize('n,emin,alpha,mode', [(6, 1e-4, .5, 'serial'), (8, 1e-4, .5, 'parallel'),
                                                (8, 1e-6, 0., 'parallel'), (16, 1e-4, .5, 'serial'),
-                                               (16, 1e-6, 1., 'parallel')])
+                                               (16, 1e-6, 1., 'parallel'), (24, 1e-4, .5, 'parallel'),
+                                               (32, 1e-6, .5, 'parallel')])
 def test_resident_matches_oracle(n, emin, alpha, mode):
     a = density(n, emin, 2 if n >= 8 else 1)
     r = pgj.Resident(n, load_mode=mode).capture()
