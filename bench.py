@@ -5,7 +5,9 @@ One step = one design step of the plain FP64 path on device-resident data:
 filter + projection -> SIMP material -> partial Green-Jacobi scale (alpha) ->
 six unit-strain PCG solves (warm-started from the previous step) with FP64
 true-residual certification -> C^H -> shear sensitivities -> OC update.
-Prints one JSON line (see the measurement contract in DESIGN.md).
+Prints one JSON line: throughput, p50/p99 step latency, the roofline of the
+dominant iteration stage (HIP events on the launch stream) and the CPU oracle
+baseline timed on a bounded sample.
 """
 import argparse
 import json

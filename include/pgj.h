@@ -231,6 +231,39 @@ int pgj_gauss_candidate(int n, const double* x, const double* ratio, const doubl
 /* bisection on the mean of the projected candidate (outer.cu:30-33). */
 int pgj_gauss_bisect(int n, const double* ws, double* bounds, double volume, void* stream);
 
+/* --------------------------------------------- generic batched PCG vectors */
+/* Vector side of the operator-agnostic best-iterate PCG on B batches of n
+ * FP64 unknowns, (B, n) row-major; the caller supplies A p and M^-1 r. One
+ * iteration is pgj_pcg_pre_step, then (unless stopping) M^-1 r and
+ * pgj_pcg_post_step. Together they replace the eager/compiled/Jacobi update
+ * paths of solve_pcg_batched
+ * (solver/src/runtime/fi/source/femlib/batched/pcg.py:20-245): the fused
+ * compiled steps (pcg.py:622-666, :741-745), the convergence latches
+ * (pcg.py:677-712), the latched best copy (pcg.py:715-738) and the Triton/CUDA Jacobi vector
+ * kernels (pcg.py:459-532). ws holds 3 B ceil(n/1024)-capped partial rows. */
+size_t pgj_pcg_ws_bytes(int B, int64_t n);
+/* r = rhs - Ax (Ax NULL: r = rhs) and rhs_norm = max(|rhs_b|, 1e-30) (pcg.py:79-85). */
+int pgj_pcg_start(int B, int64_t n, const double* rhs, const double* Ax, double* r, double* rhs_norm, double* ws,
+                  void* stream);
+/* p = z, rz = r.z per batch (pcg.py:87-89). */
+int pgj_pcg_init(int B, int64_t n, const double* r, const double* z, double* p, double* rz, double* ws,
+                 void* stream);
+/* alpha = rz/max(pAp,1e-30), 0 where pAp is not finite or <= 0; u += alpha p;
+ * r -= alpha Ap; res = |r|/rhs_norm; improved = res < best_res (latched: and
+ * state[0] == 0); best_res and best_u updated where improved. latched != 0:
+ * state (int64[2]) gets state[0] = iteration when every res < tol first holds,
+ * and state[1] = 1 when every res < near_tol first holds (near_tol > 0 only)
+ * (pcg.py:181-199, :637-666, :677-738). improved is uint8[B]. */
+int pgj_pcg_pre_step(int B, int64_t n, double* u, double* r, const double* p, const double* Ap, const double* rz,
+                     const double* rhs_norm, double* best_u, double* best_res, double* res, uint8_t* improved,
+                     int64_t* state, double tol, double near_tol, int64_t iteration, int latched, double* ws,
+                     void* stream);
+/* rz_new = r.z; p = z + rz_new/max(rz,1e-30) p; rz = rz_new (pcg.py:238-245,
+ * :741-745). minv non-NULL: z = minv * r is formed first (Jacobi,
+ * pcg.py:495-508); minv_stride is 0 (one (n) diagonal for every batch) or n. */
+int pgj_pcg_post_step(int B, int64_t n, const double* r, double* z, const double* minv, int64_t minv_stride,
+                      double* p, double* rz, double* ws, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,13 +126,15 @@ class ResidentPCG:
 
 
 def solve_pcg_batched(apply, apply_precond, rhs, *, tol=1e-6, max_iter=2000, x0=None,
-                      update_mode='eager', convergence_check_stride=1):
+                      update_mode='eager', convergence_check_stride=1, adaptive_factor=None):
     """Best-iterate batched PCG (``pcg.py:20-252``); returns (best_u, n_iter, rel_res).
 
     ``update_mode='compiled'`` with stride > 1 follows the device-latched path
     (``pcg.py:145-224``, ``:676-738``): best iterates stop updating once every
     batch is below ``tol``; the host checks at iteration 1, every ``stride``
-    iterations and at ``max_iter``.
+    iterations and at ``max_iter``. ``adaptive_factor`` (latched path only)
+    switches the host to per-iteration checks once every batch is below
+    ``adaptive_factor * tol`` (``pcg.py:689-713``, ``:205-217``).
     """
     if update_mode not in ('eager', 'compiled'):
         raise ValueError(f'unknown PCG update mode: {update_mode!r}')
@@ -152,6 +154,10 @@ def solve_pcg_batched(apply, apply_precond, rhs, *, tol=1e-6, max_iter=2000, x0=
     p = z.copy()
     rz = (r * z).sum(1)
     latched = update_mode == 'compiled' and convergence_check_stride > 1
+    if adaptive_factor is not None and not latched:
+        raise ValueError('adaptive convergence checks require the latched path')
+    host_stride = convergence_check_stride
+    near_latched = False
     best_res = np.full(B, np.inf)
     best_u = u.copy()
     stop_iter = 0
@@ -174,10 +180,14 @@ def solve_pcg_batched(apply, apply_precond, rhs, *, tol=1e-6, max_iter=2000, x0=
         best_res = np.where(improved, res, best_res)
         if latched and stop_iter == 0 and np.all(res < tol):
             stop_iter = it + 1
-        check_now = it == 0 or (it + 1) % convergence_check_stride == 0 or (it + 1) == max_iter
+        if adaptive_factor is not None and np.all(res < adaptive_factor * tol):
+            near_latched = True
+        check_now = it == 0 or (it + 1) % host_stride == 0 or (it + 1) == max_iter
         logical = 0
         if check_now:
             if latched:
+                if adaptive_factor is not None and stop_iter == 0 and near_latched:
+                    host_stride = 1
                 logical = stop_iter
             elif res.max() < tol:
                 logical = it + 1

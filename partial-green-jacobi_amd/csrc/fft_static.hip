@@ -22,8 +22,8 @@ struct Cube {
   static constexpr int KC = N == 16 ? 9 : N == 24 ? 13 : N == 32 ? 17 : N == 48 ? 5 : N == 64 ? 11 : N == 96 ? 7 : 5;
   static constexpr int NKB = NKX / KC;
   static_assert(NKX % KC == 0, "kx chunking must divide the half spectrum");
-  static constexpr size_t LDS_XY = size_t(N) * RS * 16 + 64 * 8;
-  static constexpr size_t LDS_Z = size_t(3) * N * KC * 16 + 64 * 8;
+  static constexpr size_t LDS_XY = size_t(N) * RS * 16 + 4 * 8 * 8;
+  static constexpr size_t LDS_Z = size_t(3) * N * KC * 16 + 4 * 8 * 8;
   using Rows = Lines<N, N, 0, RS, 1, false>;          // x transforms (length M) on the N rows
   using Cols = Lines<NKX, NKX, 0, 1, RS, true>;       // y transforms (length N) on the NKX columns
   using ZCols = Lines<3 * KC, KC, N * KC, 1, KC, true>;  // z transforms on (component, column)
@@ -91,11 +91,12 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgs A) {
     plane[y * C::RS + x / 2] = v;
   }
   if (MODE != kFxyScale) {
-    const double tq = block_sum(sq, red), tm = block_sum(sm, red);
+    double t[2] = {sq, sm};
+    block_sums<2>(t, red);
     if (threadIdx.x == 0) {
       double* p = A.pr + ((int64_t(l) * N + z) * 3 + c) * 2;
-      p[0] = tq;
-      p[1] = tm;
+      p[0] = t[0];
+      p[1] = t[1];
     }
   }
   __syncthreads();
@@ -125,7 +126,10 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
   constexpr int T = C::TZ, KC = C::KC, NKX = C::NKX;
   extern __shared__ double2 cols[];
   double* red = reinterpret_cast<double*>(cols + 3 * N * KC);
-  const int kb = blockIdx.x, y = blockIdx.y, l = blockIdx.z;
+  // ky is the fastest grid index: the kx chunks of one ky row (which share
+  // cache lines) get block ids N apart, a multiple of 8 for every supported N,
+  // and so run on the same XCD / L2.
+  const int y = blockIdx.x, kb = blockIdx.y, l = blockIdx.z;
   const int kx0 = kb * KC;
   const int blk = y * C::NKB + kb;
   const bool lead = threadIdx.x == 0 && kb == 0 && y == 0;
@@ -221,15 +225,14 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
     const int kcol = e % KC, cz = e / KC;
     S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
   }
-  const double trz = block_sum(rz, red);
-  double tsw[3];
-  for (int c = 0; c < 3; ++c) tsw[c] = block_sum(sw[c], red);
+  double t[4] = {rz, sw[0], sw[1], sw[2]};
+  block_sums<4>(t, red);
   if (threadIdx.x == 0) {
     double* pg = A.pg + (int64_t(l) * C::NKB * N + blk) * 4;
-    pg[0] = trz;
-    pg[1] = tsw[0];
-    pg[2] = tsw[1];
-    pg[3] = tsw[2];
+    pg[0] = t[0];
+    pg[1] = t[1];
+    pg[2] = t[2];
+    pg[3] = t[3];
   }
 }
 
@@ -327,7 +330,7 @@ int fxy_static_n(FFTArgs& A, int L, int mode, hipStream_t s) {
 template <int N>
 int fz_static_n(FFTArgs& A, int L, int mode, hipStream_t s) {
   using C = Cube<N>;
-  dim3 grid(C::NKB, N, L), block(C::TZ);
+  dim3 grid(N, C::NKB, L), block(C::TZ);
   switch (mode) {
     case kFzScale: hipLaunchKernelGGL((fz_static<N, kFzScale>), grid, block, C::LDS_Z, s, A); break;
     case kFzPre: hipLaunchKernelGGL((fz_static<N, kFzPre>), grid, block, C::LDS_Z, s, A); break;

@@ -7,28 +7,36 @@
 
 namespace pgj {
 
-// Value of the direction at a node for the fused update p = z + beta p_old.
-template <int MODE>
-__device__ __forceinline__ double node_value(const RowArgs& A, int64_t idx, double beta, bool first) {
-  if (MODE == kModeApply) return A.u[idx];
-  const double z = A.u[idx];
-  return first ? z : fma(beta, A.pold[idx], z);
-}
+// Raw inputs of one node row segment: the vector itself (apply) or the pair
+// (z, p_old) the fused update p = z + beta p_old is formed from (PCG).
+template <int MODE, int R, int XPL>
+struct Rows {
+  double v[R + 1][XPL][3];
+  double w[MODE == kModePCG ? R + 1 : 1][XPL][3];
+};
 
-template <int MODE, int XPL, int R, bool CUBIC>
-__global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
+// Row-march HEX8 apply. Lanes map to x (XPL consecutive x per lane), each of
+// the W row-waves marches R element rows over the z-layers of the block, a
+// halo wave computes the element row below the block, and contributions to a
+// neighbour wave's first node row go through a double-buffered LDS slot. The
+// next plane's inputs are fetched before the current layer is computed.
+template <int MODE, int XPL, int R, bool CUBIC, int MAXT>
+__global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
   // the drop-in apply works on AoS vectors, the resident iteration on SoA
   constexpr bool SOA = MODE == kModePCG;
   extern __shared__ double lds[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // wave index and everything derived from it are wave-uniform: keep them in SGPRs
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int l = blockIdx.z;
   const RowMarch& c = A.cfg;
   const int W = c.W;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
-  const int64_t base = int64_t(l) * 3 * A.N;
+  const unsigned N = unsigned(A.N);
   const int y0 = blockIdx.x * W * R, z0 = blockIdx.y * c.ZC;
   const int xl = lane % c.nl;
   const bool lane_on = lane < c.nl;
+  const unsigned xo = unsigned(xl * XPL);
 
   double beta = 0.0;
   bool first = false;
@@ -41,13 +49,14 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
       beta = rz_new / (rz != 0.0 ? rz : 1.0);
     }
   } else if (A.mask && !A.mask[l]) {
-    // Inactive load: the reference kernel leaves its pre-zeroed output untouched.
+    // Inactive load: zeros, as the reference's pre-zeroed output.
+    double* o = A.out + size_t(l) * 3 * N;
     if (wave < W && lane_on) {
       for (int j = 0; j < R; ++j)
         for (int zi = 0; zi < c.ZC; ++zi)
           for (int xi = 0; xi < XPL; ++xi) {
-            const int64_t node = (int64_t(z0 + zi) * ny + (y0 + wave * R + j)) * nx + xl * XPL + xi;
-            for (int d = 0; d < 3; ++d) A.out[base + 3 * node + d] = 0.0;
+            const unsigned node = unsigned((z0 + zi) * ny + (y0 + wave * R + j)) * unsigned(nx) + xo + xi;
+            for (int d = 0; d < 3; ++d) o[3 * node + d] = 0.0;
           }
     }
     return;
@@ -56,10 +65,63 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
   const bool halo = wave == W;
   const int ybase = halo ? y0 - 1 : y0 + wave * R;
   const int rows = halo ? 1 : R;  // element rows computed by this wave
+  // uniform base pointers per component (SoA) or per load (AoS)
+  const double* __restrict__ in = A.u + size_t(l) * 3 * N;
+  const double* __restrict__ old = SOA ? A.pold + size_t(l) * 3 * N : nullptr;
+  double* __restrict__ pn = SOA ? A.pnew + size_t(l) * 3 * N : nullptr;
+  double* __restrict__ out = A.out + size_t(l) * 3 * N;
+  // element offsets in bytes (32-bit): uniform base + 32-bit offset selects the SGPR-base addressing form
+  auto at = [&](unsigned node, int d) -> unsigned { return 8u * (SOA ? unsigned(d) * N + node : 3u * node + unsigned(d)); };
+  auto ld = [](const double* base, unsigned off) -> double {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+  };
+  auto st = [](double* base, unsigned off, double v) { *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off) = v; };
+  auto row_of = [&](int plane, int j) -> unsigned {
+    return unsigned(plane * ny + wrapi(ybase + j, ny)) * unsigned(nx);
+  };
+
+  auto fetch = [&](int plane, Rows<MODE, R, XPL>& dst, double (&arow)[R][XPL]) {
+#pragma unroll
+    for (int j = 0; j <= R; ++j) {
+      if (j > rows) break;
+      const unsigned rb = row_of(plane, j);
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi) {
+        const unsigned node = rb + xo + xi;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          dst.v[j][xi][d] = ld(in, at(node, d));
+          // p_old is read unconditionally (ignored right after init) so the load keeps its SGPR base
+          if constexpr (MODE == kModePCG) dst.w[j][xi][d] = ld(old, at(node, d));
+        }
+        if (j < R) arow[j][xi] = ld(A.a, 8u * node);
+      }
+    }
+  };
+  // Node values of a fetched plane; p_new is stored for owned rows.
+  auto settle = [&](int plane, const Rows<MODE, R, XPL>& src, double (&dst)[R + 1][XPL][3], bool write_p) {
+#pragma unroll
+    for (int j = 0; j <= R; ++j) {
+      if (j > rows) break;
+      const unsigned rb = row_of(plane, j);
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double v = src.v[j][xi][d];
+          if constexpr (MODE == kModePCG) {
+            if (!first) v = fma(beta, src.w[j][xi][d], v);
+            if (write_p && j < rows && !halo && lane_on) st(pn, at(rb + xo + xi, d), v);
+          }
+          dst[j][xi][d] = v;
+        }
+    }
+  };
 
   double nodeL[R + 1][XPL][3], nodeU[R + 1][XPL][3];
   double carry[R][XPL][3];
   double accL[R + 1][XPL][3], accU[R + 1][XPL][3];
+  double aL[R][XPL], aU[R][XPL], aN[R][XPL];
   double pq = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j)
@@ -68,42 +130,18 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) carry[j][xi][d] = 0.0;
 
-  auto load_plane = [&](int plane, double (&dst)[R + 1][XPL][3], bool write_p) {
-#pragma unroll
-    for (int j = 0; j <= R; ++j) {
-      if (j > rows) break;
-      const int y = wrapi(ybase + j, ny);
-#pragma unroll
-      for (int xi = 0; xi < XPL; ++xi) {
-        const int64_t node = (int64_t(plane) * ny + y) * nx + xl * XPL + xi;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const int64_t idx = SOA ? (int64_t(l) * 3 + d) * A.N + node : base + 3 * node + d;
-          const double v = node_value<MODE>(A, idx, beta, first);
-          dst[j][xi][d] = v;
-          if (MODE == kModePCG && write_p && j < rows && !halo && lane_on) A.pnew[idx] = v;
-        }
-      }
-    }
-  };
-
   const int src_right = (xl + 1) % c.nl;       // lane holding x + XPL
   const int src_left = (xl + c.nl - 1) % c.nl;  // lane holding x - 1
-  double* slot = lds + (size_t)wave * 64 * 6 * XPL;
+  const int slot_size = (c.W + 1) * 64 * 6 * XPL;
 
+  Rows<MODE, R, XPL> raw;
+  fetch(wrapi(z0 - 1, nz), raw, aL);
+  settle(wrapi(z0 - 1, nz), raw, nodeL, false);
+  fetch(z0, raw, aU);
   for (int li = 0; li <= c.ZC; ++li) {
     const int zl = wrapi(z0 - 1 + li, nz), zu = wrapi(z0 + li, nz);
-    if (li == 0) {
-      load_plane(zl, nodeL, false);
-    } else {
-#pragma unroll
-      for (int j = 0; j <= R; ++j)
-#pragma unroll
-        for (int xi = 0; xi < XPL; ++xi)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) nodeL[j][xi][d] = nodeU[j][xi][d];
-    }
-    load_plane(zu, nodeU, li < c.ZC);
+    settle(zu, raw, nodeU, li < c.ZC);
+    if (li < c.ZC) fetch(wrapi(z0 + li + 1, nz), raw, aN);  // prefetch the next plane
 #pragma unroll
     for (int j = 0; j <= R; ++j)
 #pragma unroll
@@ -116,7 +154,6 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       if (j >= rows) break;
-      const int y = wrapi(ybase + j, ny);
       // Right neighbours (x + XPL) of the four node rows this element row touches.
       double nb[2][2][3];
 #pragma unroll
@@ -129,8 +166,6 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
       double spill[2][2][3];  // contributions of this lane's last element to x + 1 (next lane)
 #pragma unroll
       for (int xi = 0; xi < XPL; ++xi) {
-        const int x = xl * XPL + xi;
-        const double ae = A.a[(int64_t(zl) * ny + y) * nx + x];
         double ue[24], fe[24];
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
@@ -147,7 +182,7 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
             ue[3 * n + d] = v;
           }
         }
-        hex8_element<CUBIC>(ue, ae, A.coef, fe);
+        hex8_element<CUBIC>(ue, aL[j][xi], A.coef, fe);
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
           const int ox = n & 1, oy = (n >> 1) & 1, oz = (n >> 2) & 1;
@@ -164,7 +199,6 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
           }
         }
       }
-      // Contributions of the left neighbour lane's last element to this lane's first x.
 #pragma unroll
       for (int oy = 0; oy < 2; ++oy)
 #pragma unroll
@@ -174,6 +208,8 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
         }
     }
     // Export the contributions to the first node row of the next wave.
+    double* slots = lds + (li & 1) * slot_size;
+    double* slot = slots + wave * 64 * 6 * XPL;
 #pragma unroll
     for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
@@ -183,7 +219,7 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
       }
     __syncthreads();
     if (!halo) {
-      const double* from = lds + (size_t)(wave == 0 ? W : wave - 1) * 64 * 6 * XPL;
+      const double* from = slots + (wave == 0 ? W : wave - 1) * 64 * 6 * XPL;
 #pragma unroll
       for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
@@ -193,16 +229,17 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
         }
       if (li >= 1 && lane_on) {
 #pragma unroll
-        for (int j = 0; j < R; ++j)
+        for (int j = 0; j < R; ++j) {
+          const unsigned rb = unsigned(zl * ny + ybase + j) * unsigned(nx);
 #pragma unroll
           for (int xi = 0; xi < XPL; ++xi) {
-            const int64_t node = (int64_t(zl) * ny + (ybase + j)) * nx + xl * XPL + xi;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-              A.out[SOA ? (int64_t(l) * 3 + d) * A.N + node : base + 3 * node + d] = accL[j][xi][d];
+              st(out, at(rb + xo + xi, d), accL[j][xi][d]);
               if (MODE == kModePCG) pq = fma(nodeL[j][xi][d], accL[j][xi][d], pq);
             }
           }
+        }
       }
 #pragma unroll
       for (int j = 0; j < R; ++j)
@@ -211,10 +248,20 @@ __global__ __launch_bounds__(576) void hex8_rowmarch(RowArgs A) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) carry[j][xi][d] = accU[j][xi][d];
     }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j <= R; ++j)
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) nodeL[j][xi][d] = nodeU[j][xi][d];
+        if (j < R) {
+          aL[j][xi] = aU[j][xi];
+          aU[j][xi] = aN[j][xi];
+        }
+      }
   }
   if (MODE == kModePCG) {
-    const double total = block_sum(pq, lds);
+    const double total = block_sum(pq, lds + 2 * slot_size);
     if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
   }
 }
@@ -229,7 +276,9 @@ __global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
   if (i < A.N) {
     const int x = int(i % A.nx), y = int((i / A.nx) % A.ny), z = int(i / (int64_t(A.nx) * A.ny));
     double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
+    // one element at a time (unrolling the eight elements spills); the node's
+    // local index in element o is o, selected with static register indices
+#pragma unroll 1
     for (int o = 0; o < 8; ++o) {
       const int ex = wrapi(x - (o & 1), A.nx), ey = wrapi(y - ((o >> 1) & 1), A.ny),
                 ez = wrapi(z - ((o >> 2) & 1), A.nz);
@@ -245,7 +294,11 @@ __global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
       const double ae = A.a[(int64_t(ez) * A.ny + ey) * A.nx + ex];
       hex8_element<CUBIC>(ue, ae, A.coef, fe);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) acc[d] += fe[3 * o + d];
+      for (int k = 0; k < 8; ++k)
+        if (k == o) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) acc[d] += fe[3 * k + d];
+        }
     }
     for (int d = 0; d < 3; ++d) {
       const int64_t idx = A.soa ? (int64_t(l) * 3 + d) * A.N + i : base + 3 * i + d;
@@ -566,31 +619,33 @@ int check_grid(const pgj_grid* g) {
   return PGJ_OK;
 }
 
-template <int MODE, int XPL, int R>
+template <int MODE, int XPL, int R, int MAXT>
 int launch_rowmarch_t(const pgj_grid& g, int L, RowArgs& A, hipStream_t s) {
   const RowMarch& c = A.cfg;
   dim3 grid(c.by, c.bz, L), block(64 * (c.W + 1));
-  const size_t lds = size_t(c.W + 1) * 64 * 6 * XPL * 8;
+  const size_t lds = (2 * size_t(c.W + 1) * 64 * 6 * XPL + 64) * 8;
   if (cubic(g))
-    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true>), grid, block, lds, s, A);
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true, MAXT>), grid, block, lds, s, A);
   else
-    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false>), grid, block, lds, s, A);
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false, MAXT>), grid, block, lds, s, A);
   PGJ_LAUNCHED("hex8_rowmarch");
   return PGJ_OK;
+}
+
+template <int MODE>
+int launch_rowmarch_m(const pgj_grid& g, int L, RowArgs& A, hipStream_t s) {
+  const RowMarch& c = A.cfg;
+  if (c.R == 2) return launch_rowmarch_t<MODE, 1, 2, 320>(g, L, A, s);
+  if (c.xpl == 1) return launch_rowmarch_t<MODE, 1, 1, 576>(g, L, A, s);
+  return launch_rowmarch_t<MODE, 2, 1, 320>(g, L, A, s);
 }
 
 int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgs& A, hipStream_t s) {
   A.nz = g.nz; A.ny = g.ny; A.nx = g.nx;
   A.N = int64_t(g.nx) * g.ny * g.nz;
-  A.cfg = rowmarch_config(g);
+  A.cfg = rowmarch_config(g, L);
   A.coef = coef_of(g);
-  const RowMarch& c = A.cfg;
-  if (mode == kModeApply) {
-    if (c.xpl == 1) return c.R == 2 ? launch_rowmarch_t<kModeApply, 1, 2>(g, L, A, s) : launch_rowmarch_t<kModeApply, 1, 1>(g, L, A, s);
-    return c.R == 2 ? launch_rowmarch_t<kModeApply, 2, 2>(g, L, A, s) : launch_rowmarch_t<kModeApply, 2, 1>(g, L, A, s);
-  }
-  if (c.xpl == 1) return c.R == 2 ? launch_rowmarch_t<kModePCG, 1, 2>(g, L, A, s) : launch_rowmarch_t<kModePCG, 1, 1>(g, L, A, s);
-  return c.R == 2 ? launch_rowmarch_t<kModePCG, 2, 2>(g, L, A, s) : launch_rowmarch_t<kModePCG, 2, 1>(g, L, A, s);
+  return mode == kModeApply ? launch_rowmarch_m<kModeApply>(g, L, A, s) : launch_rowmarch_m<kModePCG>(g, L, A, s);
 }
 
 int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_t s) {
@@ -634,7 +689,7 @@ extern "C" int pgj_hex8_apply(const pgj_grid* g, int L, const double* a, const d
                               double* out, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(L >= 1 && L <= 65535 && a && u && out, "invalid apply arguments");
-  if (!rowmarch_config(*g).valid) {
+  if (!rowmarch_config(*g, L).valid) {
     PGJ_REQUIRE(active == nullptr, "active mask needs the row-march kernel (nx <= 128 and nx %% ceil(nx/64) == 0)");
     return pgj_hex8_apply_reference(g, L, a, u, out, stream);
   }

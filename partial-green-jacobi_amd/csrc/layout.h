@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 
 #include "../../include/pgj.h"
 
@@ -26,29 +28,49 @@ struct RowMarch {
   int by, bz;       // blocks along y and z per load
 };
 
-inline RowMarch rowmarch_config(const pgj_grid& g) {
+// Shape preferences (R, W, ZC) for L loads per launch. ZC (planes marched by
+// one block) is the largest of 16, 8, 4, 2, 1 dividing nz that still gives
+// at least 192 blocks: longer marches re-read fewer halo planes, but the
+// launch must keep most of the 256 CUs busy. PGJ_HEX8_SHAPE="R,W,ZC"
+// overrides the preferences for tuning sweeps.
+inline RowMarch rowmarch_config(const pgj_grid& g, int L) {
   RowMarch c{};
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   c.xpl = (nx + 63) / 64;
-  if (c.xpl > 2 || nx % c.xpl != 0 || ny < 2 || nz < 2) return c;
+  if (c.xpl > 2 || nx % c.xpl != 0 || ny < 2 || nz < 2 || L < 1) return c;
   c.nl = nx / c.xpl;
-  c.R = 1;
+  int want_r = 1, want_w = c.xpl == 1 ? 8 : 4, want_z = 0;
+  if (const char* env = std::getenv("PGJ_HEX8_SHAPE")) {
+    int r = 0, w = 0, z = 0;
+    if (std::sscanf(env, "%d,%d,%d", &r, &w, &z) == 3 && (r == 1 || r == 2) && w >= 1 && z >= 1) {
+      want_r = r; want_w = w; want_z = z;
+    }
+  }
+  c.R = (want_r == 2 && ny % 2 == 0 && c.xpl == 1) ? 2 : 1;  // two rows of two x per lane exceed 256 VGPRs
+  // block = W row-waves + 1 halo wave: at most 5 waves (320 threads, the launch
+  // bound of the R = 2 and two-x-per-lane kernels) or 9 waves (576) otherwise
+  const int wcap = (c.R == 2 || c.xpl == 2) ? 4 : 8;
+  const int wmax = want_w < wcap ? want_w : wcap;
   const int rows = ny / c.R;
-  const int wmax = c.xpl == 1 ? 8 : 4;
   c.W = 1;
   for (int w = wmax; w >= 1; --w)
     if (rows % w == 0) { c.W = w; break; }
   c.by = ny / (c.W * c.R);
   c.ZC = 1;
-  for (int z = 4; z >= 1; --z)
-    if (nz % z == 0) { c.ZC = z; break; }
+  if (want_z > 0) {
+    for (int z = want_z; z >= 1; --z)
+      if (nz % z == 0) { c.ZC = z; break; }
+  } else {
+    for (int z = 16; z >= 1; z /= 2)
+      if (nz % z == 0 && int64_t(c.by) * (nz / z) * L >= 192) { c.ZC = z; break; }
+  }
   c.bz = nz / c.ZC;
   c.valid = 1;
   return c;
 }
 
-inline int hex8_blocks_per_load(const pgj_grid& g) {
-  RowMarch c = rowmarch_config(g);
+inline int hex8_blocks_per_load(const pgj_grid& g, int L) {
+  RowMarch c = rowmarch_config(g, L);
   if (c.valid) return c.by * c.bz;
   const int64_t n = int64_t(g.nx) * g.ny * g.nz;
   return int((n + 255) / 256);
@@ -197,7 +219,7 @@ inline ResidentLayout resident_layout(const pgj_grid& g, int L) {
   ResidentLayout r{};
   r.pre = precond_layout(g, L);
   size_t off = r.pre.bytes;
-  r.ppq = off; off = align256(off + size_t(L) * hex8_blocks_per_load(g) * 8);
+  r.ppq = off; off = align256(off + size_t(L) * hex8_blocks_per_load(g, L) * 8);
   const int64_t n = int64_t(g.nx) * g.ny * g.nz;
   r.nblk_cert = int((n + 255) / 256);
   r.pcert = off; off = align256(off + size_t(L) * r.nblk_cert * 4 * 8);

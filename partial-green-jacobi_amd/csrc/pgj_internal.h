@@ -57,6 +57,27 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
   return t;
 }
 
+// Block-wide sums of K values at once (fixed order, one pair of barriers).
+// `scratch` must hold K * blockDim.x/64 doubles; every thread receives the totals.
+template <int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) scratch[k * nw + wave] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += scratch[k * nw + w];
+    v[k] = t;
+  }
+  __syncthreads();
+}
+
 // Deterministic reduction of `count` partials at `p` (stride `stride`) by one
 // wavefront: lane-strided accumulation followed by wave_sum. Every lane of
 // every calling wavefront obtains bit-identical results.

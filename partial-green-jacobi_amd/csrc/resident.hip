@@ -153,7 +153,7 @@ extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, con
   FFTArgs A = fft_args(*g, plan);
   A.rin = r; A.s = sc; A.shat = static_cast<const double2*>(s_hat);
   A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z;
-  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g);
+  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, L);
   A.st = state_view(state, L);
   A.soa = 1;
   if (int e = fxy(A, *g, L, kFxyPre, s)) return e;
@@ -169,14 +169,14 @@ extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, co
               "invalid stage arguments");
   PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
-  PGJ_REQUIRE(rowmarch_config(*g).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
+  PGJ_REQUIRE(rowmarch_config(*g, L).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
   hipStream_t s = as_stream(stream);
   ResidentBuffers b = resident_buffers(*g, L, ws);
   StateView st = state_view(state, L);
   FFTArgs F = fft_args(*g, plan);
   F.u = u; F.r = r; F.q = q; F.p = pnew; F.s = sc; F.shat = static_cast<const double2*>(s_hat);
   F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z;
-  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g);
+  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, L);
   F.st = st;
   F.soa = 1;
   if (stage == 0) {

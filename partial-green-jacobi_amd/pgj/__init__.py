@@ -3,8 +3,10 @@
 Host-side mirror of the reference's operator / preconditioner / PCG / resident
 interfaces over the HIP C ABI in ``libpgj.so`` (declared in ``include/pgj.h``).
 """
-from . import _lib
+from . import _lib, pcg
 from .operator import Operator, build, diagonal, scale, set_alpha
+from .pcg import certified_solve, solve_pcg_batched
 from .resident import Resident
 
-__all__ = ['Operator', 'Resident', 'build', 'diagonal', 'scale', 'set_alpha']
+__all__ = ['Operator', 'Resident', 'build', 'certified_solve', 'diagonal', 'scale', 'set_alpha',
+           'solve_pcg_batched']

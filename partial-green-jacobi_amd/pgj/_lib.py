@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libpgj.so')
 
 c_void_p, c_int, c_double, c_size_t = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+c_int64 = ctypes.c_int64
 c_dp = ctypes.POINTER(ctypes.c_double)
 
 
@@ -82,6 +83,14 @@ SIGNATURES = {
     'pgj_gauss_ratio': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_gauss_candidate': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     'pgj_gauss_bisect': (c_int, [c_int, c_void_p, c_void_p, c_double, c_void_p]),
+    'pgj_pcg_ws_bytes': (c_size_t, [c_int, c_int64]),
+    'pgj_pcg_start': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_pcg_init': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_pcg_pre_step': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64,
+                                 c_int, c_void_p, c_void_p]),
+    'pgj_pcg_post_step': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                  c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,173 @@
+"""solve_pcg_batched / certified_solve on libpgj vector kernels vs the oracle (GPU).
+
+Same seeded systems on both sides: a dense SPD system with Jacobi (every
+update mode), the HEX8 operator with the partial Green-Jacobi preconditioner,
+restarts from x0, max_iter exhaustion, a zero right-hand side and the
+recycle callback schedule.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pgj
+from oracle import design, green, hex8
+from oracle import pcg as opcg
+from pgj import pcg
+
+pytestmark = pytest.mark.gpu
+LAM, MU = 15 / 26, 5 / 13
+C0 = hex8.isotropic(LAM, MU)
+
+
+def _spd(n, seed, cond=300.0):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (q * np.geomspace(1.0, cond, n)) @ q.T
+
+
+class DenseOp:
+    """Caller-side operator: A p on the device, Jacobi from diag()."""
+
+    def __init__(self, A):
+        self.A = torch.tensor(A, device='cuda')
+
+    def apply(self, p):
+        return (p @ self.A.T).contiguous()
+
+    def diag(self):
+        return torch.diagonal(self.A).contiguous()
+
+
+def _oracle_dense(A, rhs, **kw):
+    d = np.diag(A)
+    return opcg.solve_pcg_batched(lambda u: u @ A.T, lambda r: r / d, rhs, **kw)
+
+
+CASES = [('eager', 1, None), ('compiled', 1, None), ('compiled', 4, None), ('compiled', 8, 100.0),
+         ('eager', 5, None), ('triton', 3, None), ('cuda', 1, None)]
+
+
+@pytest.mark.parametrize('mode,stride,factor', CASES)
+def test_dense_jacobi_matches_oracle(mode, stride, factor):
+    A = _spd(300, 11)
+    rhs = np.random.default_rng(12).standard_normal((4, 300))
+    print(f'dense mode={mode} stride={stride} factor={factor}', flush=True)
+    u, it, res = pcg.solve_pcg_batched(DenseOp(A), torch.tensor(rhs, device='cuda'), tol=1e-9, max_iter=600,
+                                       update_mode=mode, convergence_check_stride=stride,
+                                       convergence_check_adaptive_factor=factor)
+    omode = 'compiled' if mode == 'compiled' else 'eager'
+    ou, oit, ores = _oracle_dense(A, rhs, tol=1e-9, max_iter=600, update_mode=omode,
+                                  convergence_check_stride=stride, adaptive_factor=factor)
+    # Rounding (FMA, reduction order, the host BLAS of the oracle) moves a CG residual
+    # trajectory by a few percent after ~160 iterations, so a residual within that of
+    # tol may cross it one iteration apart: the latch iteration agrees to +-1, a strided
+    # non-latched check to +-stride.
+    slack = 1 if (mode == 'compiled' and stride > 1) or stride == 1 else stride
+    assert abs(it - oit) <= slack, (it, oit)
+    if it == oit:
+        np.testing.assert_allclose(u.cpu().numpy(), ou, rtol=0, atol=1e-6 * np.abs(ou).max())
+    assert res.cpu().numpy().max() < 1e-9
+    true = np.linalg.norm(rhs - u.cpu().numpy() @ A.T, axis=1) / np.linalg.norm(rhs, axis=1)
+    assert true.max() < 1.01e-9
+
+
+def test_dense_max_iter_returns_best():
+    A = _spd(200, 5, cond=1e4)
+    rhs = np.random.default_rng(6).standard_normal((2, 200))
+    u, it, res = pcg.solve_pcg_batched(DenseOp(A), torch.tensor(rhs, device='cuda'), tol=1e-14, max_iter=25,
+                                       update_mode='compiled', convergence_check_stride=4,
+                                       verify_final_residual=True)
+    ou, oit, ores = _oracle_dense(A, rhs, tol=1e-14, max_iter=25, update_mode='compiled',
+                                  convergence_check_stride=4)
+    assert it == oit == 25
+    np.testing.assert_allclose(u.cpu().numpy(), ou, rtol=0, atol=1e-10 * np.abs(ou).max())
+    true = np.linalg.norm(rhs - ou @ A.T, axis=1) / np.linalg.norm(rhs, axis=1)
+    np.testing.assert_allclose(res.cpu().numpy(), true, rtol=1e-8)
+
+
+def test_zero_rhs_stops_at_first_iteration():
+    A = _spd(64, 7)
+    rhs = np.zeros((3, 64))
+    rhs[1] = np.random.default_rng(8).standard_normal(64)
+    u, it, res = pcg.solve_pcg_batched(DenseOp(A), torch.tensor(rhs, device='cuda'), tol=1e-8, max_iter=200)
+    ou, oit, ores = _oracle_dense(A, rhs, tol=1e-8, max_iter=200)
+    assert it == oit
+    assert np.all(u.cpu().numpy()[[0, 2]] == 0)
+    np.testing.assert_allclose(u.cpu().numpy(), ou, rtol=0, atol=1e-9 * np.abs(ou).max())
+
+
+def test_recycle_callback_schedule():
+    A = _spd(80, 9)
+    rhs = torch.tensor(np.random.default_rng(10).standard_normal((2, 80)), device='cuda')
+    seen = []
+
+    def cb(it, p, ap):
+        seen.append(it)
+        np.testing.assert_allclose((p @ torch.tensor(A, device='cuda').T).cpu().numpy(), ap.cpu().numpy(),
+                                   rtol=1e-12, atol=1e-12)
+
+    _, it, _ = pcg.solve_pcg_batched(DenseOp(A), rhs, tol=1e-10, max_iter=400, recycle_callback=cb,
+                                     recycle_stride=7)
+    assert seen == list(range(0, it, 7))
+
+
+class Hex8Op:
+    """(B, 3N) HEX8 operator with M_alpha^-1 attached, over pgj.Operator."""
+
+    def __init__(self, a, alpha):
+        self.op = pgj.operator.build(torch.tensor(a, device='cuda'))
+        pgj.operator.set_alpha(self.op, alpha)
+        self.mesh = self.op.mesh
+        self._fft_enabled = True
+
+    def apply(self, u):
+        return self.op.apply_batched(u)
+
+    def apply_precond(self, r):
+        return self.op.apply_precond(r)
+
+    def project_zero_mean(self, v):
+        return self.op.project_zero_mean(v)
+
+
+@pytest.mark.parametrize('n,alpha,mode,stride', [(8, .5, 'compiled', 4), (8, 0., 'eager', 1),
+                                                 (12, 1., 'compiled', 4), (16, .5, 'compiled', 8)])
+def test_hex8_green_matches_oracle(n, alpha, mode, stride):
+    print(f'hex8 n={n} alpha={alpha} mode={mode} stride={stride}', flush=True)
+    a = design.simp(design.filtered_noise(n, 20261104, 2), 1e-4)
+    op = Hex8Op(a, alpha)
+    f = np.concatenate([hex8.rhs(a, e, C0) for e in np.eye(6)])
+    u, it, res = pcg.solve_pcg_batched(op, torch.tensor(f, device='cuda'), tol=1e-8, max_iter=800,
+                                       update_mode=mode, convergence_check_stride=stride)
+    G6 = green.symbol(a.shape, hex8.voxel_of(a.shape), LAM, MU)
+    s = None if alpha == 0 else green.scale(a, alpha, C0)
+    ou, oit, ores = opcg.solve_pcg_batched(lambda v: hex8.apply(a, v, C0),
+                                           lambda r: green.precond(r, G6, a.shape, s), f, tol=1e-8, max_iter=800,
+                                           update_mode=mode, convergence_check_stride=stride)
+    assert abs(it - oit) <= (stride if mode == 'compiled' and stride > 1 else 1)
+    if it == oit:
+        np.testing.assert_allclose(u.cpu().numpy(), ou, rtol=0, atol=1e-7 * np.abs(ou).max())
+    true = np.linalg.norm(f - hex8.apply(a, u.cpu().numpy(), C0), axis=1) / np.linalg.norm(f, axis=1)
+    assert true.max() < 1.01e-8
+
+
+@pytest.mark.parametrize('x0', [False, True])
+def test_certified_solve_matches_backend(x0):
+    n = 8
+    a = design.simp(design.filtered_noise(n, 7, 2), 1e-4)
+    op = Hex8Op(a, .5)
+    f = hex8.rhs(a, np.eye(6)[3], C0)
+    G6 = green.symbol(a.shape, hex8.voxel_of(a.shape), LAM, MU)
+    s = green.scale(a, .5, C0)
+    start = np.random.default_rng(1).standard_normal(f.shape) * 1e-3 if x0 else None
+    u, info = pcg.certified_solve(op, torch.tensor(f, device='cuda'),
+                                  None if start is None else torch.tensor(start, device='cuda'), rtol=1e-10)
+    ou, oinfo = opcg.backend_solve(lambda v: hex8.apply(a, v, C0), lambda r: green.precond(r, G6, a.shape, s), f,
+                                   n, x0=start, rtol=1e-10)
+    assert info['converged'] and oinfo['converged']
+    assert info['restarts'] == oinfo['restarts']
+    assert abs(info['iterations'] - oinfo['iterations']) <= 4
+    assert info['absolute_residual'] <= info['threshold']
+    np.testing.assert_allclose(info['rhs_norm'], oinfo['rhs_norm'], rtol=1e-12)
+    true = np.linalg.norm(f - hex8.apply(a, u.cpu().numpy(), C0))
+    np.testing.assert_allclose(true, info['absolute_residual'], rtol=1e-6, atol=1e-20)
