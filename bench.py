@@ -73,7 +73,7 @@ def probe_stages(res, iters):
         for stage, name in enumerate(names):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            _lib.call('pgj_resident_stage', res.op.grid, res.batch, _lib.ptr(res.op._plan),
+            _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, _lib.ptr(res.op._plan),
                       _lib.ptr(res.op._material_field), _lib.ptr(res.scale), _lib.ptr(res.scale_hat),
                       _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(pold), _lib.ptr(pnew), _lib.ptr(res.q),
                       _lib.ptr(res.zv), _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())

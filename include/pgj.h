@@ -91,6 +91,15 @@ size_t pgj_scale_ws_bytes(const pgj_grid* g);
 int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alpha, double* d, double* s, double* ws,
                        void* stream);
 
+/* Row-march launch shape of the HEX8 kernel for grid g with L loads per
+ * launch: R element rows per wave (1 or 2), W row waves per block (1..8), ZC
+ * z-planes marched per block. set_shape records a tuned shape (all three 0
+ * clears it); get_shape reports the shape in use and the blocks per load
+ * (out4 = R, W, ZC, blocks). Counterpart of the reference's tile/variant
+ * sweep (solver/src/hex8_modal/modal_adapter.py:28-41, gpu_study.py:138-142). */
+int pgj_hex8_set_shape(const pgj_grid* g, int L, int R, int W, int ZC);
+int pgj_hex8_get_shape(const pgj_grid* g, int L, int* out4);
+
 /* Layout conversions between AoS (L, N, 3) and SoA (L, 3, N) vectors. */
 int pgj_aos_to_soa(const pgj_grid* g, int L, const double* in, double* out, void* stream);
 int pgj_soa_to_aos(const pgj_grid* g, int L, const double* in, double* out, void* stream);
@@ -149,30 +158,36 @@ size_t pgj_resident_ws_bytes(const pgj_grid* g, int L);
  * complex128. Replaces Resident.select (resident.py:156-174). */
 int pgj_resident_scale(const pgj_grid* g, const void* plan, const double* a, double alpha, double* s, void* s_hat,
                        void* ws, void* stream);
+/* Designs: the L loads of one resident launch form L/group consecutive
+ * designs of `group` loads each (group divides L, L <= 4096). Design d owns
+ * a[d] (N), s[d] (3, N) and s_hat[d] (3, nz, ny, nx/2+1); group = L is one
+ * design, the reference's case. Several designs of one grid in one launch is
+ * the batched form (many designs x 6 loads per kernel). */
+
 /* rhs_l for the L macro strains (host L*6), norms, targets
  * max(rtol |f|, 1e-13/sqrt(nx))^2, iterations = 0, bad = 0.
  * Replaces Resident._solve_batch set-up (resident.py:186-191). */
-int pgj_resident_setup(const pgj_grid* g, int L, const double* a, const double* macros, double rtol,
+int pgj_resident_setup(const pgj_grid* g, int L, int group, const double* a, const double* macros, double rtol,
                        double* rhs, void* state, void* ws, void* stream);
 /* u <- P u; r = rhs - K u (reference operator); absolute = |r|.
  * Replaces Resident.certify (resident.py:133-136). */
-int pgj_resident_certify(const pgj_grid* g, int L, const double* a, double* u, const double* rhs, double* r,
-                         void* state, void* ws, void* stream);
+int pgj_resident_certify(const pgj_grid* g, int L, int group, const double* a, double* u, const double* rhs,
+                         double* r, void* state, void* ws, void* stream);
 /* active = |r|^2 > target2 & iterations < 4000 & !bad; z = M^-1 r; first
  * direction armed. Replaces Resident.initialize (resident.py:119-123). */
-int pgj_resident_init(const pgj_grid* g, int L, const void* plan, const double* s, const void* s_hat,
+int pgj_resident_init(const pgj_grid* g, int L, int group, const void* plan, const double* s, const void* s_hat,
                       const double* r, double* z, void* state, void* ws, void* stream);
 /* `count` (even) masked PCG iterations; p0/p1 are ping-pong direction
  * buffers, q and z are scratch vectors, all (L, 3N). Replaces
  * Resident.iteration with update_ur/update_p (resident.py:39-58, 125-131). */
-int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* s,
+int pgj_resident_iterate(const pgj_grid* g, int L, int group, const void* plan, const double* a, const double* s,
                          const void* s_hat, double* u, double* r, double* p0, double* p1, double* q, double* z,
                          void* state, void* ws, int count, void* stream);
 /* One stage of one iteration (0: direction update + K apply, 1: vector update +
  * forward xy FFT, 2: z pass + Green, 3: inverse xy FFT + projection), reading
  * the direction from pold and writing pnew; pgj_resident_iterate is these four
  * stages with p0/p1 alternating. Exposed so each kernel can be timed alone. */
-int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, const double* a, const double* s,
+int pgj_resident_stage(const pgj_grid* g, int L, int group, const void* plan, const double* a, const double* s,
                        const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q, double* z,
                        void* state, void* ws, int stage, void* stream);
 

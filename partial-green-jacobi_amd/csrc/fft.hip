@@ -140,7 +140,7 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
     const int c = comps == 3 ? int(e % 3) : c0;
     const int64_t gnode = int64_t(z) * plane_nodes + node;
     const int64_t idx = vindex(A, l, c, gnode);
-    const int64_t sidx = vindex(A, 0, c, gnode);
+    const int64_t sidx = vindex(A, design_of(l, A.group), c, gnode);
     double val;
     if (MODE == kFxyScale) {
       val = A.s[sidx];
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
     double2 sh[3];
     if (A.shat) {
       for (int c = 0; c < 3; ++c) {
-        sh[c] = A.shat[int64_t(c) * f.nf + fidx];
+        sh[c] = A.shat[(int64_t(design_of(l, A.group)) * 3 + c) * f.nf + fidx];
         v[c] = c2(fma(-mean[c], sh[c].x, v[c].x), fma(-mean[c], sh[c].y, v[c].y));
       }
     }
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
     const double2* row = plane + (size_t(comps == 3 ? c : 0) * ny + y) * f.rs;
     const double w = even ? reinterpret_cast<const double*>(row)[x] : row[x].x;
     const int64_t gnode = int64_t(z) * plane_nodes + node;
-    const double v = A.s ? A.s[vindex(A, 0, c, gnode)] * w - mean[c] : w;
+    const double v = A.s ? A.s[vindex(A, design_of(l, A.group), c, gnode)] * w - mean[c] : w;
     A.zout[vindex(A, l, c, gnode)] = v;
   }
 }

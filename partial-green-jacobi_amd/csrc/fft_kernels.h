@@ -34,6 +34,7 @@ struct FFTArgs {
   int ppq_count;
   StateView st;
   int soa;               // vectors (and s) are SoA (L, 3, N) instead of AoS (L, N, 3)
+  int group;             // loads per design (s is (designs, 3N), s_hat (designs, 3, nf)); 0: shared
 };
 
 // Index of component c of node `node` of load l in a vector (and of s with l = 0).

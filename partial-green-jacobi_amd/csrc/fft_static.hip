@@ -52,7 +52,7 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgs A) {
     }
   }
   const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
-  const int64_t soff = int64_t(c) * NN + int64_t(z) * C::PLANE;
+  const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   double sq = 0.0, sm = 0.0;
   // two nodes (x even, x+1) per step: one packed complex of the x transform
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
     if (A.shat) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        sh[c] = A.shat[int64_t(c) * C::NF + fidx];
+        sh[c] = A.shat[(int64_t(design_of(l, A.group)) * 3 + c) * C::NF + fidx];
         v[c] = c2(fma(-mean[c], sh[c].x, v[c].x), fma(-mean[c], sh[c].y, v[c].y));
       }
     }
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgs A) {
   __syncthreads();
   sfft<true, C::M, typename C::Rows, T>(plane, A.twx, 2);
   const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
-  const int64_t soff = int64_t(c) * NN + int64_t(z) * C::PLANE;
+  const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
     const int x = e % N, y = e / N;
     const double2 w = plane[y * C::RS + x / 2];

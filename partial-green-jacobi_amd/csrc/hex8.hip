@@ -29,6 +29,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
   // wave index and everything derived from it are wave-uniform: keep them in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int l = blockIdx.z;
+  const double* __restrict__ mat = A.a + size_t(design_of(l, A.group)) * A.N;
   const RowMarch& c = A.cfg;
   const int W = c.W;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
           // p_old is read unconditionally (ignored right after init) so the load keeps its SGPR base
           if constexpr (MODE == kModePCG) dst.w[j][xi][d] = ld(old, at(node, d));
         }
-        if (j < R) arow[j][xi] = ld(A.a, 8u * node);
+        if (j < R) arow[j][xi] = ld(mat, 8u * node);
       }
     }
   };
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(256) void hex8_reference(RefArgs A) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) ue[3 * n + d] = A.u[A.soa ? (int64_t(l) * 3 + d) * A.N + node : base + 3 * node + d];
       }
-      const double ae = A.a[(int64_t(ez) * A.ny + ey) * A.nx + ex];
+      const double ae = A.a[int64_t(design_of(l, A.group)) * A.N + (int64_t(ez) * A.ny + ey) * A.nx + ex];
       hex8_element<CUBIC>(ue, ae, A.coef, fe);
 #pragma unroll
       for (int k = 0; k < 8; ++k)

@@ -26,6 +26,7 @@ struct RowArgs {
   int pg_count, pg_stride;
   double* ppq;           // per-block p.q partials, per load stride ppq_stride
   int ppq_stride;
+  int group;             // loads per design (a is (designs, N)); 0: one shared a
 };
 
 // Independent reference apply: one thread per node gathers the rows of its
@@ -43,6 +44,7 @@ struct RefArgs {
   const double* rhs;
   double* pres;
   int soa;               // vectors are SoA (L, 3, N) instead of AoS (L, N, 3)
+  int group;             // loads per design (a is (designs, N)); 0: one shared a
 };
 
 

@@ -28,11 +28,15 @@ struct RowMarch {
   int by, bz;       // blocks along y and z per load
 };
 
+// Tuned row-march shapes, keyed by (nz, ny, nx, L); set by pgj_hex8_set_shape
+// (the autotuner) and consulted by every launch. Returns false if unset.
+bool tuned_shape(const pgj_grid& g, int L, int* R, int* W, int* ZC);
+
 // Shape preferences (R, W, ZC) for L loads per launch. ZC (planes marched by
 // one block) is the largest of 16, 8, 4, 2, 1 dividing nz that still gives
 // at least 192 blocks: longer marches re-read fewer halo planes, but the
-// launch must keep most of the 256 CUs busy. PGJ_HEX8_SHAPE="R,W,ZC"
-// overrides the preferences for tuning sweeps.
+// launch must keep most of the 256 CUs busy. A tuned shape replaces the
+// preferences.
 inline RowMarch rowmarch_config(const pgj_grid& g, int L) {
   RowMarch c{};
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -40,9 +44,9 @@ inline RowMarch rowmarch_config(const pgj_grid& g, int L) {
   if (c.xpl > 2 || nx % c.xpl != 0 || ny < 2 || nz < 2 || L < 1) return c;
   c.nl = nx / c.xpl;
   int want_r = 1, want_w = c.xpl == 1 ? 8 : 4, want_z = 0;
-  if (const char* env = std::getenv("PGJ_HEX8_SHAPE")) {
+  {
     int r = 0, w = 0, z = 0;
-    if (std::sscanf(env, "%d,%d,%d", &r, &w, &z) == 3 && (r == 1 || r == 2) && w >= 1 && z >= 1) {
+    if (tuned_shape(g, L, &r, &w, &z)) {
       want_r = r; want_w = w; want_z = z;
     }
   }
@@ -74,6 +78,15 @@ inline int hex8_blocks_per_load(const pgj_grid& g, int L) {
   if (c.valid) return c.by * c.bz;
   const int64_t n = int64_t(g.nx) * g.ny * g.nz;
   return int((n + 255) / 256);
+}
+
+// Upper bound of hex8_blocks_per_load over every shape (W = ZC = R = 1), so
+// workspaces stay valid whatever shape is tuned later.
+inline int hex8_blocks_bound(const pgj_grid& g) {
+  const int64_t n = int64_t(g.nx) * g.ny * g.nz;
+  const int64_t rm = int64_t(g.ny) * g.nz;
+  const int64_t ref = (n + 255) / 256;
+  return int(rm > ref ? rm : ref);
 }
 
 // --------------------------------------------------------------------- FFT
@@ -219,13 +232,16 @@ inline ResidentLayout resident_layout(const pgj_grid& g, int L) {
   ResidentLayout r{};
   r.pre = precond_layout(g, L);
   size_t off = r.pre.bytes;
-  r.ppq = off; off = align256(off + size_t(L) * hex8_blocks_per_load(g, L) * 8);
+  r.ppq = off; off = align256(off + size_t(L) * hex8_blocks_bound(g) * 8);
   const int64_t n = int64_t(g.nx) * g.ny * g.nz;
   r.nblk_cert = int((n + 255) / 256);
   r.pcert = off; off = align256(off + size_t(L) * r.nblk_cert * 4 * 8);
   r.bytes = off;
   return r;
 }
+
+// Loads one resident launch may carry (designs x loads per design).
+constexpr int kMaxResidentLoads = 4096;
 
 // Resident state: public blocks first (documented in pgj.h), then private.
 struct StateLayout {

@@ -36,6 +36,10 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// Design of load l when consecutive groups of `group` loads share one material
+// field a, scale s and spectrum s_hat (group 0: every load shares design 0).
+__host__ __device__ __forceinline__ int design_of(int l, int group) { return group > 0 ? l / group : 0; }
+
 // Sum over the 64 lanes of a wavefront with a fixed butterfly order; every lane gets the total.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

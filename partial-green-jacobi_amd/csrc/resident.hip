@@ -83,14 +83,19 @@ extern "C" int pgj_resident_state_offsets(int L, size_t* o) {
 
 extern "C" size_t pgj_resident_ws_bytes(const pgj_grid* g, int L) { return resident_layout(*g, L).bytes; }
 
-extern "C" int pgj_resident_setup(const pgj_grid* g, int L, const double* a, const double* macros, double rtol,
-                                  double* rhs, void* state, void* ws, void* stream) {
+namespace {
+bool group_ok(int L, int group) { return L >= 1 && L <= pgj::kMaxResidentLoads && group >= 1 && L % group == 0; }
+}  // namespace
+
+extern "C" int pgj_resident_setup(const pgj_grid* g, int L, int group, const double* a, const double* macros,
+                                  double rtol, double* rhs, void* state, void* ws, void* stream) {
   if (int e = check_grid(g)) return e;
-  PGJ_REQUIRE(L >= 1 && L <= 64 && a && macros && rhs && state && ws, "invalid resident setup arguments");
+  PGJ_REQUIRE(group_ok(L, group) && a && macros && rhs && state && ws, "invalid resident setup arguments");
   const int64_t N = int64_t(g->nx) * g->ny * g->nz;
   hipStream_t s = as_stream(stream);
   for (int l = 0; l < L; ++l)
-    if (int e = launch_rhs(*g, a, macros + 6 * l, rhs + int64_t(l) * 3 * N, 1, s)) return e;
+    if (int e = launch_rhs(*g, a + int64_t(l / group) * N, macros + 6 * l, rhs + int64_t(l) * 3 * N, 1, s))
+      return e;
   ResidentBuffers b = resident_buffers(*g, L, ws);
   const int nblk = b.lay.nblk_cert;
   hipLaunchKernelGGL(square_partials, dim3(nblk, L), dim3(256), 0, s, rhs, N, b.pcert);
@@ -101,16 +106,16 @@ extern "C" int pgj_resident_setup(const pgj_grid* g, int L, const double* a, con
   return PGJ_OK;
 }
 
-extern "C" int pgj_resident_certify(const pgj_grid* g, int L, const double* a, double* u, const double* rhs,
-                                    double* r, void* state, void* ws, void* stream) {
+extern "C" int pgj_resident_certify(const pgj_grid* g, int L, int group, const double* a, double* u,
+                                    const double* rhs, double* r, void* state, void* ws, void* stream) {
   if (int e = check_grid(g)) return e;
-  PGJ_REQUIRE(L >= 1 && L <= 64 && a && u && rhs && r && state && ws, "invalid certify arguments");
+  PGJ_REQUIRE(group_ok(L, group) && a && u && rhs && r && state && ws, "invalid certify arguments");
   hipStream_t s = as_stream(stream);
   ResidentBuffers b = resident_buffers(*g, L, ws);
   if (int e = launch_component_sums(*g, L, u, b.pcert, 1, s)) return e;
   if (int e = launch_subtract_mean(*g, L, u, b.pcert, 1, s)) return e;
   RefArgs A{};
-  A.a = a; A.u = u; A.out = r; A.rhs = rhs; A.pres = b.pcert; A.soa = 1;
+  A.a = a; A.u = u; A.out = r; A.rhs = rhs; A.pres = b.pcert; A.soa = 1; A.group = group;
   if (int e = launch_reference(*g, L, A, true, s)) return e;
   hipLaunchKernelGGL(certify_finish, dim3(L), dim3(64), 0, s, b.pcert, b.lay.nblk_cert, state_view(state, L));
   PGJ_LAUNCHED("certify_finish");
@@ -143,10 +148,10 @@ extern "C" int pgj_resident_scale(const pgj_grid* g, const void* plan, const dou
   return fz(A, *g, 1, kFzScale, s);
 }
 
-extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, const double* sc, const void* s_hat,
-                                 const double* r, double* z, void* state, void* ws, void* stream) {
+extern "C" int pgj_resident_init(const pgj_grid* g, int L, int group, const void* plan, const double* sc,
+                                 const void* s_hat, const double* r, double* z, void* state, void* ws, void* stream) {
   if (int e = check_fft(g)) return e;
-  PGJ_REQUIRE(L >= 1 && L <= 64 && plan && r && z && state && ws, "invalid init arguments");
+  PGJ_REQUIRE(group_ok(L, group) && plan && r && z && state && ws, "invalid init arguments");
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   hipStream_t s = as_stream(stream);
   ResidentBuffers b = resident_buffers(*g, L, ws);
@@ -156,16 +161,17 @@ extern "C" int pgj_resident_init(const pgj_grid* g, int L, const void* plan, con
   A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, L);
   A.st = state_view(state, L);
   A.soa = 1;
+  A.group = group;
   if (int e = fxy(A, *g, L, kFxyPre, s)) return e;
   if (int e = fz(A, *g, L, kFzInit, s)) return e;
   return fxyi(A, *g, L, s);
 }
 
-extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
-                                  const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q,
-                                  double* z, void* state, void* ws, int stage, void* stream) {
+extern "C" int pgj_resident_stage(const pgj_grid* g, int L, int group, const void* plan, const double* a,
+                                  const double* sc, const void* s_hat, double* u, double* r, double* pold,
+                                  double* pnew, double* q, double* z, void* state, void* ws, int stage, void* stream) {
   if (int e = check_fft(g)) return e;
-  PGJ_REQUIRE(L >= 1 && L <= 64 && plan && a && u && r && pold && pnew && q && z && state && ws,
+  PGJ_REQUIRE(group_ok(L, group) && plan && a && u && r && pold && pnew && q && z && state && ws,
               "invalid stage arguments");
   PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
@@ -179,12 +185,14 @@ extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, co
   F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, L);
   F.st = st;
   F.soa = 1;
+  F.group = group;
   if (stage == 0) {
     const int pg_count = F.f.nkb * g->ny;
     RowArgs K{};
     K.a = a; K.u = z; K.out = q; K.pold = pold; K.pnew = pnew; K.st = st;
     K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
     K.ppq = b.ppq; K.ppq_stride = F.ppq_count;
+    K.group = group;
     return launch_rowmarch(*g, L, kModePCG, K, s);
   }
   if (stage == 1) return fxy(F, *g, L, kFxyPCG, s);
@@ -192,15 +200,17 @@ extern "C" int pgj_resident_stage(const pgj_grid* g, int L, const void* plan, co
   return fxyi(F, *g, L, s);
 }
 
-extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, const void* plan, const double* a, const double* sc,
-                                    const void* s_hat, double* u, double* r, double* p0, double* p1, double* q,
-                                    double* z, void* state, void* ws, int count, void* stream) {
+extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, const void* plan, const double* a,
+                                    const double* sc, const void* s_hat, double* u, double* r, double* p0,
+                                    double* p1, double* q, double* z, void* state, void* ws, int count,
+                                    void* stream) {
   PGJ_REQUIRE(count >= 2 && count % 2 == 0, "iteration count must be even and >= 2, got %d", count);
   for (int i = 0; i < count; ++i) {
     double* pold = (i % 2 == 0) ? p0 : p1;
     double* pnew = (i % 2 == 0) ? p1 : p0;
     for (int stage = 0; stage < 4; ++stage)
-      if (int e = pgj_resident_stage(g, L, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws, stage, stream))
+      if (int e = pgj_resident_stage(g, L, group, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws, stage,
+                                     stream))
         return e;
   }
   return PGJ_OK;

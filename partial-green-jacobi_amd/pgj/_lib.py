@@ -39,6 +39,8 @@ SIGNATURES = {
     'pgj_hex8_avg_stress': (c_int, [GP, c_void_p, c_void_p, c_dp, c_void_p, c_void_p, c_void_p]),
     'pgj_scale_ws_bytes': (c_size_t, [GP]),
     'pgj_diagonal_scale': (c_int, [GP, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hex8_set_shape': (c_int, [GP, c_int, c_int, c_int, c_int]),
+    'pgj_hex8_get_shape': (c_int, [GP, c_int, ctypes.POINTER(c_int)]),
     'pgj_aos_to_soa': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p]),
     'pgj_soa_to_aos': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p]),
     'pgj_resident_scale': (c_int, [GP, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -54,16 +56,16 @@ SIGNATURES = {
     'pgj_resident_state_bytes': (c_size_t, [c_int]),
     'pgj_resident_state_offsets': (c_int, [c_int, ctypes.POINTER(c_size_t)]),
     'pgj_resident_ws_bytes': (c_size_t, [GP, c_int]),
-    'pgj_resident_setup': (c_int, [GP, c_int, c_void_p, c_dp, c_double, c_void_p, c_void_p, c_void_p,
+    'pgj_resident_setup': (c_int, [GP, c_int, c_int, c_void_p, c_dp, c_double, c_void_p, c_void_p, c_void_p,
                                    c_void_p]),
-    'pgj_resident_certify': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                     c_void_p]),
-    'pgj_resident_init': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+    'pgj_resident_certify': (c_int, [GP, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
+    'pgj_resident_init': (c_int, [GP, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
-    'pgj_resident_iterate': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+    'pgj_resident_iterate': (c_int, [GP, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                      c_void_p]),
-    'pgj_resident_stage': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+    'pgj_resident_stage': (c_int, [GP, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     'pgj_oc_ws_bytes': (c_size_t, [c_int]),
     'pgj_cone_filter': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),

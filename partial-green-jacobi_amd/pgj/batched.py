@@ -1,0 +1,241 @@
+"""Batched form: many designs x 6 loads of uneven grids in one captured graph.
+
+Designs of the same grid size n form a ``DesignGroup``: D designs x G loads
+are one resident launch of L = D*G loads (libpgj's ``group`` argument), each
+design with its own SIMP field, scale s and spectrum s_hat, all loads with
+their own PCG scalars and masks. ``BatchedDesigns`` holds one group per grid
+size and captures the init / iteration-chunk / certify steps of every group
+into one graph each, so one replay advances every design by ``chunk``
+iterations. The host reads the active masks of all groups with one copy per
+chunk. The per-load recurrence, masks, cap, targets and restarts are those
+of the plain resident solver (``exp/asg_gpu_resident_20261103/resident.py:39-58``,
+``:119-136``, ``:186-209``); only the launch is shared.
+
+``tune`` picks the HEX8 row-march shape of every group by timing the
+candidates on the group's own buffers (``pgj_hex8_set_shape``).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
+
+LOADS6 = np.eye(6)
+
+
+class DesignGroup:
+    """D designs of one periodic n^3 grid, each solved for the same G macro loads."""
+
+    def __init__(self, n, designs, loads=LOADS6, chunk=4, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda'):
+        if chunk < 2 or chunk % 2:
+            raise ValueError('graph chunk must be even and >= 2')
+        self.loads = np.atleast_2d(np.asarray(loads, dtype=np.float64))
+        self.n, self.D, self.G, self.chunk = int(n), int(designs), len(self.loads), chunk
+        self.L = self.D * self.G
+        shape = (self.n,) * 3
+        self.shape = shape
+        self.op = Operator(torch.ones(shape, dtype=torch.float64, device=device), None, lam, mu, device).enable_green()
+        g = self.op.grid
+        N = self.n ** 3
+        self.N = N
+        L = self.L
+        dev = torch.device(device)
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        self.a = torch.ones((self.D, N), dtype=torch.float64, device=dev)
+        self.scale = torch.ones((self.D, 3 * N), dtype=torch.float64, device=dev)
+        nf = self.n * self.n * (self.n // 2 + 1)
+        self.scale_hat = torch.zeros((self.D, 3 * nf), dtype=torch.complex128, device=dev)
+        self.alpha = np.zeros(self.D)
+        # SoA (L, 3, N) solver vectors; u persists between solves as the warm start
+        self.u, self.r, self.p0, self.p1, self.q, self.zv, self.rhs = (z(L, 3 * N) for _ in range(7))
+        lib = _lib.lib()
+        self.state = torch.zeros(lib.pgj_resident_state_bytes(L), dtype=torch.uint8, device=dev)
+        offs = (_lib.c_size_t * 7)()
+        _lib.check(lib.pgj_resident_state_offsets(L, offs))
+        o = list(offs)
+        self.norm = self.state[o[2]:o[2] + 8 * L].view(torch.float64)
+        self.absolute = self.state[o[3]:o[3] + 8 * L].view(torch.float64)
+        self.target2 = self.state[o[1]:o[1] + 8 * L].view(torch.float64)
+        self.iterations = self.state[o[4]:o[4] + 8 * L].view(torch.int64)
+        self.active = self.state[o[5]:o[5] + L]
+        self.bad = self.state[o[6]:o[6] + L]
+        self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, L), dtype=torch.uint8, device=dev)
+        self.scale_ws = torch.empty(lib.pgj_resident_ws_bytes(g, 1), dtype=torch.uint8, device=dev)
+        self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
+        self.aos = z(self.G, 3 * N)
+
+    # -- per-design inputs ----------------------------------------------------------
+    def material(self, d, a):
+        """SIMP field of design d (n, n, n)."""
+        self.a[d].copy_(torch.as_tensor(a, dtype=torch.float64).reshape(-1))
+
+    def select(self, d, alpha):
+        """Partial Green-Jacobi exponent of design d: s_d and its spectrum (``resident.py:156-174``)."""
+        _lib.call('pgj_resident_scale', self.op.grid, _lib.ptr(self.op._plan), _lib.ptr(self.a[d]), float(alpha),
+                  _lib.ptr(self.scale[d]), _lib.ptr(self.scale_hat[d]), _lib.ptr(self.scale_ws), _lib.stream())
+        self.alpha[d] = float(alpha)
+
+    # -- launches (all designs at once) --------------------------------------------
+    def setup(self, rtol):
+        m = np.tile(self.loads, (self.D, 1)).reshape(-1)
+        _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, _lib.ptr(self.a), _lib.host_doubles(m),
+                  float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
+
+    def certify(self):
+        _lib.call('pgj_resident_certify', self.op.grid, self.L, self.G, _lib.ptr(self.a), _lib.ptr(self.u),
+                  _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
+
+    def initialize(self):
+        _lib.call('pgj_resident_init', self.op.grid, self.L, self.G, _lib.ptr(self.op._plan), _lib.ptr(self.scale),
+                  _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(self.ws), _lib.stream())
+
+    def iteration(self, count=None):
+        _lib.call('pgj_resident_iterate', self.op.grid, self.L, self.G, _lib.ptr(self.op._plan), _lib.ptr(self.a),
+                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
+                  _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(self.ws), int(count or self.chunk), _lib.stream())
+
+    def stage(self, k, pold, pnew):
+        _lib.call('pgj_resident_stage', self.op.grid, self.L, self.G, _lib.ptr(self.op._plan), _lib.ptr(self.a),
+                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
+                  _lib.ptr(pold), _lib.ptr(pnew), _lib.ptr(self.q), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(self.ws), int(k), _lib.stream())
+
+    # -- results ----------------------------------------------------------------------
+    def solutions(self, d):
+        """AoS (G, 3N) solutions of design d."""
+        _lib.call('pgj_soa_to_aos', self.op.grid, self.G, _lib.ptr(self.u[d * self.G:(d + 1) * self.G]),
+                  _lib.ptr(self.aos), _lib.stream())
+        return self.aos.clone()
+
+    def effective_tensor(self, d):
+        """C^H (6, 6) of design d (``reproduction.py:166-176``)."""
+        if self.G != 6 or not np.array_equal(self.loads, LOADS6):
+            raise ValueError('C^H needs the six unit Voigt strains')
+        _lib.call('pgj_soa_to_aos', self.op.grid, self.G, _lib.ptr(self.u[d * self.G:(d + 1) * self.G]),
+                  _lib.ptr(self.aos), _lib.stream())
+        out = torch.empty(36, dtype=torch.float64, device=self.a.device)
+        _lib.call('pgj_effective_tensor', self.op.grid, _lib.ptr(self.a[d]), _lib.ptr(self.aos), _lib.ptr(self.ch_ws),
+                  _lib.ptr(out), _lib.stream())
+        return out.reshape(6, 6)
+
+    def shape_info(self):
+        out = (_lib.c_int * 4)()
+        _lib.check(_lib.lib().pgj_hex8_get_shape(self.op.grid, self.L, out))
+        return tuple(out)
+
+
+class BatchedDesigns:
+    """Designs of several grid sizes, one ``DesignGroup`` per size, advanced together by one graph.
+
+    ``sizes`` maps n to the number of designs of that size.
+    """
+
+    def __init__(self, sizes, loads=LOADS6, chunk=4, device='cuda'):
+        self.groups = {int(n): DesignGroup(n, d, loads, chunk, device=device) for n, d in sorted(sizes.items())}
+        self.chunk = chunk
+        self.init_graph = self.iter_graph = self.cert_graph = None
+
+    def _each(self, name):
+        def run():
+            for g in self.groups.values():
+                getattr(g, name)()
+        return run
+
+    def tune(self, reps=6, candidates=None):
+        """Time row-march shapes per group on its own buffers and keep the fastest; returns the choices."""
+        chosen = {}
+        for n, g in self.groups.items():
+            grid = g.op.grid
+            lib = _lib.lib()
+            if candidates is None:
+                cand = [(1, w, zc) for w in (2, 4, 8) for zc in (1, 2, 4, 8, 16) if n % zc == 0]
+            else:
+                cand = list(candidates)
+            g.setup(1e-8)
+            g.r.normal_()
+            g.certify()
+            g.initialize()
+            g.stage(0, g.p0, g.p1)
+            best = None
+            for shape in cand:
+                _lib.check(lib.pgj_hex8_set_shape(grid, g.L, *shape))
+                g.stage(0, g.p0, g.p1)   # warm
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    g.stage(0, g.p0, g.p1)
+                e1.record()
+                torch.cuda.synchronize()
+                t = e0.elapsed_time(e1) / reps
+                if best is None or t < best[0]:
+                    best = (t, shape)
+            _lib.check(lib.pgj_hex8_set_shape(grid, g.L, *best[1]))
+            chosen[n] = dict(shape=best[1], ms=best[0])
+            g.u.zero_()
+        torch.cuda.synchronize()
+        return chosen
+
+    def capture(self):
+        """Capture init / chunk / certify for every group into one graph each."""
+        for g in self.groups.values():
+            g.setup(1e-8)
+            g.r.normal_()
+        self._each('certify')()
+        self._each('initialize')()
+        self._each('iteration')()
+        torch.cuda.synchronize()
+        for name, fn in (('init_graph', self._each('initialize')), ('iter_graph', self._each('iteration')),
+                         ('cert_graph', self._each('certify'))):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                fn()
+            setattr(self, name, graph)
+        torch.cuda.synchronize()
+        for g in self.groups.values():
+            g.u.zero_()
+        return self
+
+    def _replay(self, graph, name):
+        if graph is None:
+            self._each(name)()
+        else:
+            graph.replay()
+
+    def _any_active(self):
+        return bool(torch.cat([g.active for g in self.groups.values()]).any().item())
+
+    def solve(self, rtol=1e-8):
+        """Solve every load of every design from the current warm starts; returns per-group info lists.
+
+        Restarts (at most two) and the failure rule follow ``resident.py:186-209`` per load.
+        """
+        for g in self.groups.values():
+            g.setup(rtol)
+        restarts = 0
+        host_checks = 0
+        self._replay(self.cert_graph, 'certify')
+        while True:
+            self._replay(self.init_graph, 'initialize')
+            while True:
+                host_checks += 1
+                if not self._any_active():
+                    break
+                self._replay(self.iter_graph, 'iteration')
+            self._replay(self.cert_graph, 'certify')
+            diag = {n: torch.stack((g.norm, g.absolute, g.target2.sqrt(), g.iterations.to(torch.float64),
+                                    g.bad.to(torch.float64)), dim=1).cpu().numpy() for n, g in self.groups.items()}
+            ok = {n: np.isfinite(v[:, 1]) & (v[:, 1] <= v[:, 2]) & (v[:, 4] == 0) for n, v in diag.items()}
+            all_ok = all(o.all() for o in ok.values())
+            hopeless = any(np.any(v[:, 4]) or np.any((~ok[n]) & (v[:, 3] >= 4000)) for n, v in diag.items())
+            if all_ok or restarts >= 2 or hopeless:
+                break
+            restarts += 1
+        infos = {n: [dict(rhs_norm=float(v[0]), absolute_residual=float(v[1]), threshold=float(v[2]),
+                          iterations=int(v[3]), relative_residual=float(v[1] / max(v[0], 1e-300)),
+                          converged=bool(ok[n][j]), restarts=restarts) for j, v in enumerate(rows)]
+                 for n, rows in diag.items()}
+        if not all_ok:
+            raise RuntimeError(('true residual certification failed', infos))
+        return infos, host_checks

@@ -1,0 +1,85 @@
+"""Batched form: designs of uneven grids in one captured graph vs solo solves and the oracle (GPU).
+
+Every design gets its own seeded filtered-noise density and alpha; each
+design's C^H and per-load iteration counts must match a solo parallel-mode
+solve of the same design (C^H within 1e-10, iterations within 1), and one
+small design is checked against the oracle. Sizes cover the runtime-shaped
+FFT path (8, 12) and the compile-time one (16, 24).
+"""
+import numpy as np
+import pytest
+import torch
+
+import pgj
+from pgj.batched import BatchedDesigns
+from oracle import design, green, hex8, pcg
+
+pytestmark = pytest.mark.gpu
+LAM, MU = 15 / 26, 5 / 13
+C0 = hex8.isotropic(LAM, MU)
+ALPHAS = (0.25, 0.5, 0.75)
+
+
+def field(n, k):
+    return design.simp(design.filtered_noise(n, 20261104 + 97 * n + k, 2 if n >= 12 else 1), 1e-4)
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize('graphs,tuned', [(True, True), (False, False)])
+def test_batched_matches_solo(graphs, tuned):
+    sizes = {8: 2, 12: 2, 16: 3, 24: 2}
+    print(f'batched sizes={sizes} graphs={graphs} tuned={tuned}', flush=True)
+    bd = BatchedDesigns(sizes)
+    if tuned:
+        bd.tune(reps=2)
+    if graphs:
+        bd.capture()
+    for n, g in bd.groups.items():
+        for d in range(g.D):
+            g.material(d, torch.tensor(field(n, d), device='cuda'))
+            g.select(d, ALPHAS[(n + d) % 3])
+    infos, _ = bd.solve(rtol=1e-8)
+    for n, g in bd.groups.items():
+        for d in range(g.D):
+            solo = pgj.Resident(n, load_mode='parallel', batch=6).capture()
+            solo.material(torch.tensor(field(n, d), device='cuda'))
+            solo.select(ALPHAS[(n + d) % 3])
+            sinfo, _ = solo.solve(rtol=1e-8)
+            for j in range(6):
+                got = infos[n][6 * d + j]
+                assert got['converged'] and got['relative_residual'] <= 1e-8
+                assert abs(got['iterations'] - sinfo[j]['iterations']) <= 1, (n, d, j, got, sinfo[j])
+            assert rel(g.effective_tensor(d), solo.effective_tensor().cpu().numpy()) < 1e-10, (n, d)
+
+
+def test_batched_design_matches_oracle():
+    bd = BatchedDesigns({8: 3}).capture()
+    g = bd.groups[8]
+    for d in range(3):
+        g.material(d, torch.tensor(field(8, d), device='cuda'))
+        g.select(d, ALPHAS[d])
+    infos, _ = bd.solve(rtol=1e-10)
+    for d in range(3):
+        a = field(8, d)
+        ref = pcg.ResidentPCG(a, C0, np.eye(6), batch=6, s=green.scale(a, ALPHAS[d], C0))
+        rinfo, _ = ref.solve(rtol=1e-10)
+        for j in range(6):
+            assert abs(infos[8][6 * d + j]['iterations'] - rinfo[j]['iterations']) <= 1
+        CH = design.effective_tensor(a, ref.solutions, C0)
+        assert rel(g.effective_tensor(d), CH) < 1e-10
+
+
+def test_warm_start_second_solve_is_cheaper():
+    bd = BatchedDesigns({16: 2}).capture()
+    g = bd.groups[16]
+    for d in range(2):
+        g.material(d, torch.tensor(field(16, d), device='cuda'))
+        g.select(d, .5)
+    first, _ = bd.solve(rtol=1e-8)
+    second, _ = bd.solve(rtol=1e-8)
+    assert all(i['iterations'] == 0 for i in second[16])
+    assert sum(i['iterations'] for i in first[16]) > 0
