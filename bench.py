@@ -43,6 +43,15 @@ def parse():
     p.add_argument('--probe-iters', type=int, default=10)
     p.add_argument('--cpu-seconds', type=float, default=15.0)
     p.add_argument('--no-cpu', action='store_true')
+    p.add_argument('--form', choices=('plain', 'batched', 'serve'), default='plain',
+                   help='plain: the FP64 drop-in design step (default); batched: 48 designs of uneven grids in one '
+                        'captured graph vs the same designs solved one at a time; serve: Poisson-arrival '
+                        'design-step requests through the continuous-batching server')
+    p.add_argument('--serve-seconds', type=float, default=20.0, help='arrival window of the serve form')
+    p.add_argument('--serve-rate', type=float, default=20.0)
+    p.add_argument('--serve-clients', type=int, default=64)
+    p.add_argument('--serve-sizes', default='32,48,64,96,128')
+    p.add_argument('--batched-per-size', type=int, default=12)
     return p.parse_args()
 
 
@@ -73,7 +82,7 @@ def probe_stages(res, iters):
         for stage, name in enumerate(names):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-            _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, _lib.ptr(res.op._plan),
+            _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, 0, res.batch, _lib.ptr(res.op._plan),
                       _lib.ptr(res.op._material_field), _lib.ptr(res.scale), _lib.ptr(res.scale_hat),
                       _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(pold), _lib.ptr(pnew), _lib.ptr(res.q),
                       _lib.ptr(res.zv), _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())
@@ -120,12 +129,124 @@ def cpu_baseline(a, alpha, its_total, seconds):
                       f'iterations plus 12 certification applies'}
 
 
+def batched_form(args):
+    """48 designs (12 each of 24^3, 32^3, 48^3, 64^3), Emin 1e-4, alpha from {0.25, 0.5, 0.75}: one graph vs solo."""
+    import pgj
+    from pgj.batched import BatchedDesigns
+    from pgj.density import filtered_noise, simp
+
+    sizes = {24: args.batched_per_size, 32: args.batched_per_size, 48: args.batched_per_size,
+             64: args.batched_per_size}
+    rng = np.random.default_rng(args.seed)
+    designs = {n: [(simp(filtered_noise(n, args.seed + 7 * n + d, 2 if n < 48 else 4), 1e-4),
+                    float(rng.choice([0.25, 0.5, 0.75]))) for d in range(D)] for n, D in sizes.items()}
+    total = sum(sizes.values())
+    bd = BatchedDesigns(sizes)
+    tuned = bd.tune(reps=3)
+    bd.capture()
+
+    def load():
+        for n, g in bd.groups.items():
+            g.u.zero_()
+            for d, (a, alpha) in enumerate(designs[n]):
+                g.material(d, torch.tensor(a, device='cuda'))
+                g.select(d, alpha)
+
+    load()
+    bd.solve(rtol=1e-8)                                   # warm-up (graphs, caches)
+    times = []
+    for _ in range(max(1, args.steps // 4)):
+        load()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos, checks = bd.solve(rtol=1e-8)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t_batched = min(times)
+    its = sum(i['iterations'] for v in infos.values() for i in v)
+    worst = max(i['relative_residual'] for v in infos.values() for i in v)
+    CH = {n: [g.effective_tensor(d).cpu().numpy() for d in range(g.D)] for n, g in bd.groups.items()}
+    # the plain drop-in on the same designs, one at a time (parallel load mode, one group of 6)
+    solo_time, solo_its, ch_err = 0.0, 0, 0.0
+    for n in sizes:
+        res = pgj.Resident(n, np.eye(6), 'parallel', batch=6).capture()
+        for d, (a, alpha) in enumerate(designs[n]):
+            res.solutions.zero_()
+            res.material(torch.tensor(a, device='cuda'))
+            res.select(alpha)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sinfo, _ = res.solve(rtol=1e-8)
+            torch.cuda.synchronize()
+            solo_time += time.perf_counter() - t0
+            solo_its += sum(i['iterations'] for i in sinfo)
+            H = res.effective_tensor().cpu().numpy()
+            ch_err = max(ch_err, float(np.abs(CH[n][d] - H).max() / np.abs(H).max()))
+    out = {
+        'metric': 'certified 6-load solves/s, batched form (48 designs of uneven grids in one captured graph)',
+        'value': total / t_batched, 'unit': 'certified 6-load solves/s', 'n_gpus': 1, 'steps': len(times),
+        'warmup': 1, 'ms_per_step': 1e3 * t_batched, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded filtered-noise densities)',
+        'config': {'workload': f'{total} designs: ' + ', '.join(f'{D} x {n}^3' for n, D in sizes.items()) +
+                               ', Emin 1e-4, alpha in {0.25, 0.5, 0.75}, 6 unit-strain loads each, zero start, '
+                               'rtol 1e-8', 'designs': total, 'rtol': 1e-8},
+        'plain_same_designs': {'value': total / solo_time, 'seconds': solo_time, 'iterations': solo_its},
+        'speedup_vs_plain': solo_time / t_batched,
+        'iterations': its, 'host_checks': checks, 'max_relative_residual': worst,
+        'ch_vs_solo_max_rel': ch_err,
+        'tuned_shapes': {str(n): v['shape'] for n, v in tuned.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_form(args):
+    """Poisson design-step requests from many clients through the continuous-batching server."""
+    from pgj.serve import Server, latency_stats, make_clients, poisson_arrivals
+
+    sizes = [int(v) for v in args.serve_sizes.split(',')]
+    clients = make_clients(args.serve_clients, sizes, (1e-4, 1e-6, 1e-9), 50, args.seed)
+    arrivals = poisson_arrivals(args.serve_rate, args.serve_seconds, [c.steps for c in clients], args.seed)
+    free, total = torch.cuda.mem_get_info()
+    budget = 0.6 * free
+    server = Server(sizes, budget, max_slots=16)
+    done = server.run(clients, arrivals, horizon=args.serve_seconds, deadline=2 * args.serve_seconds)
+    elapsed = server.elapsed
+    stats = latency_stats(done)
+    rel = max((h['max_rel'] for c in clients for h in c.history), default=0.0)
+    by_size = {}
+    for r in done:
+        by_size.setdefault(r.client.n, []).append(r.end - r.arrival)
+    out = {
+        'metric': 'design-step requests/s through the continuous-batching server, with p50/p99 latency',
+        'value': len(done) / elapsed, 'unit': 'certified design steps/s', 'n_gpus': 1, 'steps': len(done),
+        'warmup': 0, 'ms_per_step': 1e3 * elapsed / max(len(done), 1), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded clients)',
+        'config': {'workload': f'{len(clients)} clients, grids {sizes}, Emin in (1e-4, 1e-6, 1e-9), alpha U[0,1], '
+                               f'Poisson arrivals {args.serve_rate:g}/s for {args.serve_seconds:g} s, 6 loads, '
+                               f'FP64 certified at 1e-8', 'arrivals': len(arrivals)},
+        'latency_ms': {'p50': 1e3 * stats['p50'], 'p99': 1e3 * stats['p99'], 'mean': 1e3 * stats['mean']},
+        'latency_ms_by_size': {str(n): {'p50': 1e3 * float(np.percentile(v, 50)),
+                                        'p99': 1e3 * float(np.percentile(v, 99)), 'count': len(v)}
+                               for n, v in sorted(by_size.items())},
+        'unserved': server.unserved, 'max_relative_residual': rel,
+        'slots': {str(n): g.D for n, g in server.groups.items()},
+        'resident_gib': sum(server.memory().values()) / 2 ** 30,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU scaling is not part of it',
                           'n_gpus': args.gpus}))
         sys.exit(2)
+    if args.form == 'batched':
+        torch.cuda.set_device(0)
+        return batched_form(args)
+    if args.form == 'serve':
+        torch.cuda.set_device(0)
+        return serve_form(args)
     import pgj
     from pgj.density import filtered_noise
     from pgj.outer import PaperOuter

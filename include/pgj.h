@@ -158,38 +158,42 @@ size_t pgj_resident_ws_bytes(const pgj_grid* g, int L);
  * complex128. Replaces Resident.select (resident.py:156-174). */
 int pgj_resident_scale(const pgj_grid* g, const void* plan, const double* a, double alpha, double* s, void* s_hat,
                        void* ws, void* stream);
-/* Designs: the L loads of one resident launch form L/group consecutive
+/* Resident sets: the L loads of a resident set form L/group consecutive
  * designs of `group` loads each (group divides L, L <= 4096). Design d owns
  * a[d] (N), s[d] (3, N) and s_hat[d] (3, nz, ny, nx/2+1); group = L is one
- * design, the reference's case. Several designs of one grid in one launch is
- * the batched form (many designs x 6 loads per kernel). */
+ * design, the reference's case, and several designs of one grid in one
+ * launch is the batched form. Vectors, `state` and `ws` are laid out for all
+ * L loads; each call below works on the slice of loads [l0, l0 + nl), which
+ * must cover whole designs, and leaves every other load untouched, so designs
+ * can join and leave a running set (continuous batching). Pass l0 = 0,
+ * nl = L for the whole set. All pointers are to the whole set's buffers. */
 
 /* rhs_l for the L macro strains (host L*6), norms, targets
  * max(rtol |f|, 1e-13/sqrt(nx))^2, iterations = 0, bad = 0.
  * Replaces Resident._solve_batch set-up (resident.py:186-191). */
-int pgj_resident_setup(const pgj_grid* g, int L, int group, const double* a, const double* macros, double rtol,
-                       double* rhs, void* state, void* ws, void* stream);
+int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, int nl, const double* a, const double* macros,
+                       double rtol, double* rhs, void* state, void* ws, void* stream);
 /* u <- P u; r = rhs - K u (reference operator); absolute = |r|.
  * Replaces Resident.certify (resident.py:133-136). */
-int pgj_resident_certify(const pgj_grid* g, int L, int group, const double* a, double* u, const double* rhs,
-                         double* r, void* state, void* ws, void* stream);
+int pgj_resident_certify(const pgj_grid* g, int L, int group, int l0, int nl, const double* a, double* u,
+                         const double* rhs, double* r, void* state, void* ws, void* stream);
 /* active = |r|^2 > target2 & iterations < 4000 & !bad; z = M^-1 r; first
  * direction armed. Replaces Resident.initialize (resident.py:119-123). */
-int pgj_resident_init(const pgj_grid* g, int L, int group, const void* plan, const double* s, const void* s_hat,
-                      const double* r, double* z, void* state, void* ws, void* stream);
-/* `count` (even) masked PCG iterations; p0/p1 are ping-pong direction
+int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* s,
+                      const void* s_hat, const double* r, double* z, void* state, void* ws, void* stream);
+/* `iters` (even) masked PCG iterations; p0/p1 are ping-pong direction
  * buffers, q and z are scratch vectors, all (L, 3N). Replaces
  * Resident.iteration with update_ur/update_p (resident.py:39-58, 125-131). */
-int pgj_resident_iterate(const pgj_grid* g, int L, int group, const void* plan, const double* a, const double* s,
-                         const void* s_hat, double* u, double* r, double* p0, double* p1, double* q, double* z,
-                         void* state, void* ws, int count, void* stream);
+int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* a,
+                         const double* s, const void* s_hat, double* u, double* r, double* p0, double* p1, double* q,
+                         double* z, void* state, void* ws, int iters, void* stream);
 /* One stage of one iteration (0: direction update + K apply, 1: vector update +
  * forward xy FFT, 2: z pass + Green, 3: inverse xy FFT + projection), reading
  * the direction from pold and writing pnew; pgj_resident_iterate is these four
  * stages with p0/p1 alternating. Exposed so each kernel can be timed alone. */
-int pgj_resident_stage(const pgj_grid* g, int L, int group, const void* plan, const double* a, const double* s,
-                       const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q, double* z,
-                       void* state, void* ws, int stage, void* stream);
+int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* a,
+                       const double* s, const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q,
+                       double* z, void* state, void* ws, int stage, void* stream);
 
 /* ------------------------------------------------------------ OC updates */
 

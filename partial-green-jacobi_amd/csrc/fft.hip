@@ -120,7 +120,7 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
   double alpha = 0.0;
   if (MODE == kFxyPCG) {
     if (!A.st.active[l]) return;
-    const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_count, A.ppq_count, 1);
+    const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
     const double rz = wave_reduce_partials(A.pg + int64_t(l) * f.nkb * ny * 4, f.nkb * ny, 4);
     const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
     alpha = valid ? rz / pq : 0.0;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
       }
     } else {
       const bool active = A.st.active[l] != 0;
-      const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_count, A.ppq_count, 1);
+      const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
       const double rz = A.st.rz[l];
       const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
       const bool finite = isfinite(res2);

@@ -30,8 +30,8 @@ struct FFTArgs {
   double* zout;          // fxyi output
   double* pr;            // fxy partials  [l][z][c][2] = (sum r^2, sum r)
   double* pg;            // fz partials   [l][blk][4]  = (v.w, s.w_x, s.w_y, s.w_z)
-  const double* ppq;     // hex8 partials [l][blk]
-  int ppq_count;
+  const double* ppq;     // hex8 partials [l][blk], ppq_count blocks in rows of ppq_stride
+  int ppq_count, ppq_stride;
   StateView st;
   int soa;               // vectors (and s) are SoA (L, 3, N) instead of AoS (L, N, 3)
   int group;             // loads per design (s is (designs, 3N), s_hat (designs, 3, nf)); 0: shared

@@ -285,22 +285,23 @@ struct StateView {
   double* alpha;
 };
 
-inline StateView state_view(void* base, int L) {
+// View of loads [l0, ...) of a state block laid out for L loads.
+inline StateView state_view(void* base, int L, int l0 = 0) {
   StateLayout s = state_layout(L);
   char* b = static_cast<char*>(base);
   StateView v;
-  v.rz = reinterpret_cast<double*>(b + s.rz);
-  v.target2 = reinterpret_cast<double*>(b + s.target2);
-  v.norm = reinterpret_cast<double*>(b + s.norm);
-  v.absolute = reinterpret_cast<double*>(b + s.absolute);
-  v.it = reinterpret_cast<int64_t*>(b + s.it);
-  v.active = reinterpret_cast<uint8_t*>(b + s.active);
-  v.bad = reinterpret_cast<uint8_t*>(b + s.bad);
-  v.it_next = reinterpret_cast<int64_t*>(b + s.it_next);
-  v.active_next = reinterpret_cast<uint8_t*>(b + s.active_next);
-  v.bad_next = reinterpret_cast<uint8_t*>(b + s.bad_next);
-  v.first = reinterpret_cast<uint8_t*>(b + s.first);
-  v.alpha = reinterpret_cast<double*>(b + s.alpha);
+  v.rz = reinterpret_cast<double*>(b + s.rz) + l0;
+  v.target2 = reinterpret_cast<double*>(b + s.target2) + l0;
+  v.norm = reinterpret_cast<double*>(b + s.norm) + l0;
+  v.absolute = reinterpret_cast<double*>(b + s.absolute) + l0;
+  v.it = reinterpret_cast<int64_t*>(b + s.it) + l0;
+  v.active = reinterpret_cast<uint8_t*>(b + s.active) + l0;
+  v.bad = reinterpret_cast<uint8_t*>(b + s.bad) + l0;
+  v.it_next = reinterpret_cast<int64_t*>(b + s.it_next) + l0;
+  v.active_next = reinterpret_cast<uint8_t*>(b + s.active_next) + l0;
+  v.bad_next = reinterpret_cast<uint8_t*>(b + s.bad_next) + l0;
+  v.first = reinterpret_cast<uint8_t*>(b + s.first) + l0;
+  v.alpha = reinterpret_cast<double*>(b + s.alpha) + l0;
   return v;
 }
 

@@ -56,17 +56,27 @@ struct ResidentBuffers {
   double* pg;
 };
 
-ResidentBuffers resident_buffers(const pgj_grid& g, int L, void* ws) {
+// Workspace regions of a resident set of L loads; the per-load rows start at load l0.
+ResidentBuffers resident_buffers(const pgj_grid& g, int L, void* ws, int l0 = 0) {
   ResidentBuffers b{};
   b.lay = resident_layout(g, L);
+  const FFTGeom f = fft_geom(g);
   char* w = static_cast<char*>(ws);
-  b.ppq = reinterpret_cast<double*>(w + b.lay.ppq);
-  b.pcert = reinterpret_cast<double*>(w + b.lay.pcert);
-  b.spec = reinterpret_cast<double2*>(w + b.lay.pre.spec);
-  b.pr = reinterpret_cast<double*>(w + b.lay.pre.pr);
-  b.pg = reinterpret_cast<double*>(w + b.lay.pre.pg);
+  b.ppq = reinterpret_cast<double*>(w + b.lay.ppq) + int64_t(l0) * hex8_blocks_bound(g);
+  b.pcert = reinterpret_cast<double*>(w + b.lay.pcert) + int64_t(l0) * b.lay.nblk_cert * 4;
+  b.spec = reinterpret_cast<double2*>(w + b.lay.pre.spec) + int64_t(l0) * 3 * f.nf;
+  b.pr = reinterpret_cast<double*>(w + b.lay.pre.pr) + int64_t(l0) * g.nz * 6;
+  b.pg = reinterpret_cast<double*>(w + b.lay.pre.pg) + int64_t(l0) * f.nkb * g.ny * 4;
   return b;
 }
+
+// Loads [l0, l0 + nl) of a resident set of L loads in designs of `group` loads.
+struct Slice {
+  int L, group, l0, nl;
+  int64_t N, nf;
+  int64_t vec() const { return int64_t(l0) * 3 * N; }          // offset into (L, 3N) vectors
+  int64_t design() const { return int64_t(l0 / group); }      // first design of the slice
+};
 
 }  // namespace pgj
 
@@ -84,40 +94,58 @@ extern "C" int pgj_resident_state_offsets(int L, size_t* o) {
 extern "C" size_t pgj_resident_ws_bytes(const pgj_grid* g, int L) { return resident_layout(*g, L).bytes; }
 
 namespace {
-bool group_ok(int L, int group) { return L >= 1 && L <= pgj::kMaxResidentLoads && group >= 1 && L % group == 0; }
+int make_slice(const pgj_grid* g, int L, int group, int l0, int nl, Slice* out) {
+  PGJ_REQUIRE(L >= 1 && L <= kMaxResidentLoads && group >= 1 && L % group == 0,
+              "resident set needs 1 <= L <= %d loads in designs of `group` loads, got L=%d group=%d",
+              kMaxResidentLoads, L, group);
+  PGJ_REQUIRE(l0 >= 0 && nl >= 1 && l0 + nl <= L && l0 % group == 0 && nl % group == 0,
+              "load slice [%d, %d) must cover whole designs of %d loads within %d", l0, l0 + nl, group, L);
+  out->L = L; out->group = group; out->l0 = l0; out->nl = nl;
+  out->N = int64_t(g->nx) * g->ny * g->nz;
+  out->nf = int64_t(g->nz) * g->ny * (g->nx / 2 + 1);
+  return PGJ_OK;
+}
 }  // namespace
 
-extern "C" int pgj_resident_setup(const pgj_grid* g, int L, int group, const double* a, const double* macros,
-                                  double rtol, double* rhs, void* state, void* ws, void* stream) {
+extern "C" int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, int nl, const double* a,
+                                  const double* macros, double rtol, double* rhs, void* state, void* ws,
+                                  void* stream) {
   if (int e = check_grid(g)) return e;
-  PGJ_REQUIRE(group_ok(L, group) && a && macros && rhs && state && ws, "invalid resident setup arguments");
-  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  Slice sl{};
+  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(a && macros && rhs && state && ws, "invalid resident setup arguments");
+  const int64_t N = sl.N;
   hipStream_t s = as_stream(stream);
-  for (int l = 0; l < L; ++l)
-    if (int e = launch_rhs(*g, a + int64_t(l / group) * N, macros + 6 * l, rhs + int64_t(l) * 3 * N, 1, s))
+  double* f = rhs + sl.vec();
+  for (int l = 0; l < nl; ++l)
+    if (int e = launch_rhs(*g, a + (sl.design() + l / group) * N, macros + 6 * l, f + int64_t(l) * 3 * N, 1, s))
       return e;
-  ResidentBuffers b = resident_buffers(*g, L, ws);
+  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
   const int nblk = b.lay.nblk_cert;
-  hipLaunchKernelGGL(square_partials, dim3(nblk, L), dim3(256), 0, s, rhs, N, b.pcert);
+  hipLaunchKernelGGL(square_partials, dim3(nblk, nl), dim3(256), 0, s, f, N, b.pcert);
   PGJ_LAUNCHED("square_partials");
-  hipLaunchKernelGGL(setup_finish, dim3(L), dim3(64), 0, s, b.pcert, nblk, rtol, 1e-13 / sqrt(double(g->nx)),
-                     state_view(state, L));
+  hipLaunchKernelGGL(setup_finish, dim3(nl), dim3(64), 0, s, b.pcert, nblk, rtol, 1e-13 / sqrt(double(g->nx)),
+                     state_view(state, L, l0));
   PGJ_LAUNCHED("setup_finish");
   return PGJ_OK;
 }
 
-extern "C" int pgj_resident_certify(const pgj_grid* g, int L, int group, const double* a, double* u,
-                                    const double* rhs, double* r, void* state, void* ws, void* stream) {
+extern "C" int pgj_resident_certify(const pgj_grid* g, int L, int group, int l0, int nl, const double* a,
+                                    double* u, const double* rhs, double* r, void* state, void* ws, void* stream) {
   if (int e = check_grid(g)) return e;
-  PGJ_REQUIRE(group_ok(L, group) && a && u && rhs && r && state && ws, "invalid certify arguments");
+  Slice sl{};
+  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(a && u && rhs && r && state && ws, "invalid certify arguments");
   hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, L, ws);
-  if (int e = launch_component_sums(*g, L, u, b.pcert, 1, s)) return e;
-  if (int e = launch_subtract_mean(*g, L, u, b.pcert, 1, s)) return e;
+  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
+  double* us = u + sl.vec();
+  if (int e = launch_component_sums(*g, nl, us, b.pcert, 1, s)) return e;
+  if (int e = launch_subtract_mean(*g, nl, us, b.pcert, 1, s)) return e;
   RefArgs A{};
-  A.a = a; A.u = u; A.out = r; A.rhs = rhs; A.pres = b.pcert; A.soa = 1; A.group = group;
-  if (int e = launch_reference(*g, L, A, true, s)) return e;
-  hipLaunchKernelGGL(certify_finish, dim3(L), dim3(64), 0, s, b.pcert, b.lay.nblk_cert, state_view(state, L));
+  A.a = a + sl.design() * sl.N; A.u = us; A.out = r + sl.vec(); A.rhs = rhs + sl.vec(); A.pres = b.pcert;
+  A.soa = 1; A.group = group;
+  if (int e = launch_reference(*g, nl, A, true, s)) return e;
+  hipLaunchKernelGGL(certify_finish, dim3(nl), dim3(64), 0, s, b.pcert, b.lay.nblk_cert, state_view(state, L, l0));
   PGJ_LAUNCHED("certify_finish");
   return PGJ_OK;
 }
@@ -148,69 +176,79 @@ extern "C" int pgj_resident_scale(const pgj_grid* g, const void* plan, const dou
   return fz(A, *g, 1, kFzScale, s);
 }
 
-extern "C" int pgj_resident_init(const pgj_grid* g, int L, int group, const void* plan, const double* sc,
-                                 const void* s_hat, const double* r, double* z, void* state, void* ws, void* stream) {
+extern "C" int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
+                                 const double* sc, const void* s_hat, const double* r, double* z, void* state,
+                                 void* ws, void* stream) {
   if (int e = check_fft(g)) return e;
-  PGJ_REQUIRE(group_ok(L, group) && plan && r && z && state && ws, "invalid init arguments");
+  Slice sl{};
+  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(plan && r && z && state && ws, "invalid init arguments");
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, L, ws);
+  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
   FFTArgs A = fft_args(*g, plan);
-  A.rin = r; A.s = sc; A.shat = static_cast<const double2*>(s_hat);
-  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z;
-  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, L);
-  A.st = state_view(state, L);
+  A.rin = r + sl.vec();
+  A.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
+  A.shat = s_hat ? static_cast<const double2*>(s_hat) + sl.design() * 3 * sl.nf : nullptr;
+  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z + sl.vec();
+  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, nl); A.ppq_stride = hex8_blocks_bound(*g);
+  A.st = state_view(state, L, l0);
   A.soa = 1;
   A.group = group;
-  if (int e = fxy(A, *g, L, kFxyPre, s)) return e;
-  if (int e = fz(A, *g, L, kFzInit, s)) return e;
-  return fxyi(A, *g, L, s);
+  if (int e = fxy(A, *g, nl, kFxyPre, s)) return e;
+  if (int e = fz(A, *g, nl, kFzInit, s)) return e;
+  return fxyi(A, *g, nl, s);
 }
 
-extern "C" int pgj_resident_stage(const pgj_grid* g, int L, int group, const void* plan, const double* a,
-                                  const double* sc, const void* s_hat, double* u, double* r, double* pold,
-                                  double* pnew, double* q, double* z, void* state, void* ws, int stage, void* stream) {
+extern "C" int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
+                                  const double* a, const double* sc, const void* s_hat, double* u, double* r,
+                                  double* pold, double* pnew, double* q, double* z, void* state, void* ws, int stage,
+                                  void* stream) {
   if (int e = check_fft(g)) return e;
-  PGJ_REQUIRE(group_ok(L, group) && plan && a && u && r && pold && pnew && q && z && state && ws,
-              "invalid stage arguments");
+  Slice sl{};
+  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(plan && a && u && r && pold && pnew && q && z && state && ws, "invalid stage arguments");
   PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
-  PGJ_REQUIRE(rowmarch_config(*g, L).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
+  PGJ_REQUIRE(rowmarch_config(*g, nl).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
   hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, L, ws);
-  StateView st = state_view(state, L);
+  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
+  StateView st = state_view(state, L, l0);
+  const int64_t v = sl.vec();
   FFTArgs F = fft_args(*g, plan);
-  F.u = u; F.r = r; F.q = q; F.p = pnew; F.s = sc; F.shat = static_cast<const double2*>(s_hat);
-  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z;
-  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, L);
+  F.u = u + v; F.r = r + v; F.q = q + v; F.p = pnew + v;
+  F.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
+  F.shat = s_hat ? static_cast<const double2*>(s_hat) + sl.design() * 3 * sl.nf : nullptr;
+  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z + v;
+  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl); F.ppq_stride = hex8_blocks_bound(*g);
   F.st = st;
   F.soa = 1;
   F.group = group;
   if (stage == 0) {
     const int pg_count = F.f.nkb * g->ny;
     RowArgs K{};
-    K.a = a; K.u = z; K.out = q; K.pold = pold; K.pnew = pnew; K.st = st;
+    K.a = a + sl.design() * sl.N; K.u = z + v; K.out = q + v; K.pold = pold + v; K.pnew = pnew + v; K.st = st;
     K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
-    K.ppq = b.ppq; K.ppq_stride = F.ppq_count;
+    K.ppq = b.ppq; K.ppq_stride = F.ppq_stride;
     K.group = group;
-    return launch_rowmarch(*g, L, kModePCG, K, s);
+    return launch_rowmarch(*g, nl, kModePCG, K, s);
   }
-  if (stage == 1) return fxy(F, *g, L, kFxyPCG, s);
-  if (stage == 2) return fz(F, *g, L, kFzPCG, s);
-  return fxyi(F, *g, L, s);
+  if (stage == 1) return fxy(F, *g, nl, kFxyPCG, s);
+  if (stage == 2) return fz(F, *g, nl, kFzPCG, s);
+  return fxyi(F, *g, nl, s);
 }
 
-extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, const void* plan, const double* a,
-                                    const double* sc, const void* s_hat, double* u, double* r, double* p0,
-                                    double* p1, double* q, double* z, void* state, void* ws, int count,
+extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
+                                    const double* a, const double* sc, const void* s_hat, double* u, double* r,
+                                    double* p0, double* p1, double* q, double* z, void* state, void* ws, int iters,
                                     void* stream) {
-  PGJ_REQUIRE(count >= 2 && count % 2 == 0, "iteration count must be even and >= 2, got %d", count);
-  for (int i = 0; i < count; ++i) {
+  PGJ_REQUIRE(iters >= 2 && iters % 2 == 0, "iteration count must be even and >= 2, got %d", iters);
+  for (int i = 0; i < iters; ++i) {
     double* pold = (i % 2 == 0) ? p0 : p1;
     double* pnew = (i % 2 == 0) ? p1 : p0;
     for (int stage = 0; stage < 4; ++stage)
-      if (int e = pgj_resident_stage(g, L, group, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws, stage,
-                                     stream))
+      if (int e = pgj_resident_stage(g, L, group, l0, nl, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws,
+                                     stage, stream))
         return e;
   }
   return PGJ_OK;

@@ -75,30 +75,40 @@ class DesignGroup:
                   _lib.ptr(self.scale[d]), _lib.ptr(self.scale_hat[d]), _lib.ptr(self.scale_ws), _lib.stream())
         self.alpha[d] = float(alpha)
 
-    # -- launches (all designs at once) --------------------------------------------
-    def setup(self, rtol):
-        m = np.tile(self.loads, (self.D, 1)).reshape(-1)
-        _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, _lib.ptr(self.a), _lib.host_doubles(m),
+    # -- launches: designs [d0, d0 + nd) (default: all) -------------------------------
+    def _slice(self, d0, nd):
+        nd = self.D - d0 if nd is None else nd
+        if d0 < 0 or nd < 1 or d0 + nd > self.D:
+            raise ValueError(f'design range [{d0}, {d0 + nd}) outside 0..{self.D}')
+        return d0 * self.G, nd * self.G
+
+    def setup(self, rtol, d0=0, nd=None):
+        l0, nl = self._slice(d0, nd)
+        m = np.tile(self.loads, (nl // self.G, 1)).reshape(-1)
+        _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.a), _lib.host_doubles(m),
                   float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
 
-    def certify(self):
-        _lib.call('pgj_resident_certify', self.op.grid, self.L, self.G, _lib.ptr(self.a), _lib.ptr(self.u),
+    def certify(self, d0=0, nd=None):
+        l0, nl = self._slice(d0, nd)
+        _lib.call('pgj_resident_certify', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.a), _lib.ptr(self.u),
                   _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
 
-    def initialize(self):
-        _lib.call('pgj_resident_init', self.op.grid, self.L, self.G, _lib.ptr(self.op._plan), _lib.ptr(self.scale),
-                  _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv), _lib.ptr(self.state),
-                  _lib.ptr(self.ws), _lib.stream())
+    def initialize(self, d0=0, nd=None):
+        l0, nl = self._slice(d0, nd)
+        _lib.call('pgj_resident_init', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.op._plan),
+                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv),
+                  _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
 
-    def iteration(self, count=None):
-        _lib.call('pgj_resident_iterate', self.op.grid, self.L, self.G, _lib.ptr(self.op._plan), _lib.ptr(self.a),
-                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
+    def iteration(self, count=None, d0=0, nd=None):
+        l0, nl = self._slice(d0, nd)
+        _lib.call('pgj_resident_iterate', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.op._plan),
+                  _lib.ptr(self.a), _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
                   _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.zv), _lib.ptr(self.state),
                   _lib.ptr(self.ws), int(count or self.chunk), _lib.stream())
 
     def stage(self, k, pold, pnew):
-        _lib.call('pgj_resident_stage', self.op.grid, self.L, self.G, _lib.ptr(self.op._plan), _lib.ptr(self.a),
-                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
+        _lib.call('pgj_resident_stage', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.op._plan),
+                  _lib.ptr(self.a), _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
                   _lib.ptr(pold), _lib.ptr(pnew), _lib.ptr(self.q), _lib.ptr(self.zv), _lib.ptr(self.state),
                   _lib.ptr(self.ws), int(k), _lib.stream())
 

@@ -105,27 +105,28 @@ class Resident:
 
     def _setup(self, load_indices, rtol):
         m = self.loads[list(load_indices)].reshape(-1)
-        _lib.call('pgj_resident_setup', self.op.grid, len(load_indices), len(load_indices),
-                  _lib.ptr(self.op._material_field),
+        nl = len(load_indices)
+        _lib.call('pgj_resident_setup', self.op.grid, nl, nl, 0, nl, _lib.ptr(self.op._material_field),
                   _lib.host_doubles(m), float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
                   _lib.stream())
 
     def certify(self):
         """u <- P u; r = f - K u; absolute = |r| (``resident.py:133-136``)."""
-        _lib.call('pgj_resident_certify', self.op.grid, self.batch, self.batch, _lib.ptr(self.op._material_field),
+        _lib.call('pgj_resident_certify', self.op.grid, self.batch, self.batch, 0, self.batch,
+                  _lib.ptr(self.op._material_field),
                   _lib.ptr(self.u), _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws),
                   _lib.stream())
 
     def initialize(self):
         """active mask and first direction z = M^-1 r (``resident.py:119-123``)."""
-        _lib.call('pgj_resident_init', self.op.grid, self.batch, self.batch, _lib.ptr(self.op._plan),
+        _lib.call('pgj_resident_init', self.op.grid, self.batch, self.batch, 0, self.batch, _lib.ptr(self.op._plan),
                   _lib.ptr(self.scale),
                   _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv), _lib.ptr(self.state),
                   _lib.ptr(self.ws), _lib.stream())
 
     def iteration(self, count=None):
         """``count`` fused masked PCG iterations (``resident.py:125-131``)."""
-        _lib.call('pgj_resident_iterate', self.op.grid, self.batch, self.batch, _lib.ptr(self.op._plan),
+        _lib.call('pgj_resident_iterate', self.op.grid, self.batch, self.batch, 0, self.batch, _lib.ptr(self.op._plan),
                   _lib.ptr(self.op._material_field), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
                   _lib.ptr(self.u), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
                   _lib.ptr(self.zv), _lib.ptr(self.state), _lib.ptr(self.ws), int(count or self.chunk),

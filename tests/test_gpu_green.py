@@ -58,3 +58,27 @@ def test_precond_properties_at_64():
     s = green.scale(a, .5, C0)
     ref = green.precond(u[:1].cpu().numpy(), green.symbol((n,) * 3, (1 / n,) * 3, LAM, MU), (n,) * 3, s)
     assert rel(Mu[:1].cpu().numpy(), ref) < 1e-13
+
+
+@pytest.mark.parametrize('n', [16, 24, 32, 48, 64, 96, 128])
+def test_static_kernels_match_runtime(n):
+    """The compile-time-shaped resident FFT kernels agree with the runtime-shaped ones (and the oracle to 48^3)."""
+    print(f'static vs runtime n={n}', flush=True)
+    a = design.simp(design.filtered_noise(n, 5 + n, 2), 1e-4)
+    res = pgj.Resident(n, load_mode='parallel', batch=3)
+    res.material(torch.tensor(a, device='cuda'))
+    res.select(0.5)
+    s_aos = res.to_aos(res.scale)
+    shat_rt = res.op.scale_spectrum(s_aos)
+    assert rel(res.scale_hat.cpu().numpy().reshape(-1), shat_rt.cpu().numpy().reshape(-1)) < 1e-13
+    rng = np.random.default_rng(n)
+    r = torch.tensor(rng.standard_normal((3, 3 * n ** 3)), device='cuda')
+    res._setup(range(3), 1e-8)
+    res.r.copy_(res.to_soa(r))
+    res.initialize()
+    z = res.to_aos(res.zv).cpu().numpy()
+    ref = res.op.green_precond(r, s_aos, shat_rt).cpu().numpy()
+    assert rel(z, ref) < 1e-12
+    if n <= 48:
+        G6 = green.symbol(a.shape, hex8.voxel_of(a.shape), LAM, MU)
+        assert rel(z, green.precond(r.cpu().numpy(), G6, a.shape, green.scale(a, 0.5, C0))) < 1e-12
