@@ -529,7 +529,12 @@ int launch_fxyi(FFTArgs& A, int L, hipStream_t s) {
   return PGJ_OK;
 }
 
+#define PGJ_RUNTIME_FFT(A)                                                                              \
+  PGJ_REQUIRE((A).f.runtime_ok, "grid (%d,%d,%d) has only the compile-time resident FFT path", (A).nz, (A).ny, \
+              (A).nx)
+
 int launch_fxy_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A);
   switch (mode) {
     case kFxyScale: return launch_fxy<kFxyScale>(A, L, s);
     case kFxyPre: return launch_fxy<kFxyPre>(A, L, s);
@@ -538,6 +543,7 @@ int launch_fxy_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
 }
 
 int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A);
   switch (mode) {
     case kFzScale: return launch_fz<kFzScale>(A, L, s);
     case kFzPre: return launch_fz<kFzPre>(A, L, s);
@@ -547,6 +553,7 @@ int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
 }
 
 int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A);
   return mode == kFxyiPre ? launch_fxyi<kFxyiPre>(A, L, s) : launch_fxyi<kFxyiPCG>(A, L, s);
 }
 

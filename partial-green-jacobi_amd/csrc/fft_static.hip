@@ -350,13 +350,7 @@ int fxyi_static_n(FFTArgs& A, int L, hipStream_t s) {
   return PGJ_OK;
 }
 
-bool static_fft(const pgj_grid& g) {
-  if (g.nx != g.ny || g.ny != g.nz) return false;
-  switch (g.nx) {
-    case 16: case 24: case 32: case 48: case 64: case 96: case 128: return true;
-    default: return false;
-  }
-}
+bool static_fft(const pgj_grid& g) { return static_cube(g); }
 
 template <int N> struct FxyCall { static int go(FFTArgs& A, int L, int m, hipStream_t s) { return fxy_static_n<N>(A, L, m, s); } };
 template <int N> struct FzCall { static int go(FFTArgs& A, int L, int m, hipStream_t s) { return fz_static_n<N>(A, L, m, s); } };

@@ -121,8 +121,18 @@ struct FFTGeom {
   int threads_xy, threads_z;
   size_t lds_xy, lds_z;
   Factors fx, fy, fz;
-  int valid;
+  int valid;        // some FFT path supports the grid
+  int runtime_ok;   // the runtime-shaped kernels (fft.hip) support it
 };
+
+// Cubic grids with compile-time-shaped resident FFT kernels (fft_static.hip).
+inline bool static_cube(const pgj_grid& g) {
+  if (g.nx != g.ny || g.ny != g.nz) return false;
+  switch (g.nx) {
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128: return true;
+    default: return false;
+  }
+}
 
 constexpr size_t kLdsLimit = 160 * 1024 - 2048;
 constexpr int kStageValues = 16;  // complex values a thread holds in one in-place LDS stage
@@ -180,6 +190,8 @@ inline FFTGeom fft_geom(const pgj_grid& g) {
     f.valid = stages_fit(f.fx, rows, f.threads_xy) && stages_fit(f.fy, f.comps * f.nkx, f.threads_xy) &&
               stages_fit(f.fz, 3 * f.kc, f.threads_z);
   }
+  f.runtime_ok = f.valid;
+  if (static_cube(g)) f.valid = 1;
   return f;
 }
 

@@ -39,7 +39,8 @@ class Resident:
         self.batch = 1 if load_mode == 'serial' else (batch or min(3, nl))
         if nl % self.batch:
             raise ValueError('batch must divide the number of loads')
-        self.load_mode, self.n, self.chunk = load_mode, int(n), chunk
+        self.load_mode, self.chunk = load_mode, chunk
+        self.n = int(n) if np.isscalar(n) else tuple(int(v) for v in n)
         shape = (n, n, n) if np.isscalar(n) else tuple(n)
         self.shape = shape
         self.op = Operator(torch.ones(shape, dtype=torch.float64, device=device), None, lam, mu, device).enable_green()

@@ -69,16 +69,20 @@ def test_static_kernels_match_runtime(n):
     res.material(torch.tensor(a, device='cuda'))
     res.select(0.5)
     s_aos = res.to_aos(res.scale)
-    shat_rt = res.op.scale_spectrum(s_aos)
-    assert rel(res.scale_hat.cpu().numpy().reshape(-1), shat_rt.cpu().numpy().reshape(-1)) < 1e-13
     rng = np.random.default_rng(n)
     r = torch.tensor(rng.standard_normal((3, 3 * n ** 3)), device='cuda')
     res._setup(range(3), 1e-8)
     res.r.copy_(res.to_soa(r))
     res.initialize()
     z = res.to_aos(res.zv).cpu().numpy()
-    ref = res.op.green_precond(r, s_aos, shat_rt).cpu().numpy()
-    assert rel(z, ref) < 1e-12
-    if n <= 48:
+    if n < 128:   # 128^3 planes exceed the runtime kernels' register budget: compile-time path only
+        shat_rt = res.op.scale_spectrum(s_aos)
+        assert rel(res.scale_hat.cpu().numpy().reshape(-1), shat_rt.cpu().numpy().reshape(-1)) < 1e-13
+        ref = res.op.green_precond(r, s_aos, shat_rt).cpu().numpy()
+        assert rel(z, ref) < 1e-12
+    else:
+        with pytest.raises(ValueError, match='compile-time'):
+            res.op.scale_spectrum(s_aos)
+    if n <= 48 or n == 128:
         G6 = green.symbol(a.shape, hex8.voxel_of(a.shape), LAM, MU)
         assert rel(z, green.precond(r.cpu().numpy(), G6, a.shape, green.scale(a, 0.5, C0))) < 1e-12
