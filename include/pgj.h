@@ -195,6 +195,42 @@ int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, int nl, cons
                        const double* s, const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q,
                        double* z, void* state, void* ws, int stage, void* stream);
 
+/* ------------------------------------------ mixed precision (FP32 inner) */
+/* FP32 inner PCG inside FP64 iterative refinement. The inner solver is the
+ * resident iteration in FP32 (row-march HEX8, compile-time Stockham FFT and
+ * Green multiply; FP64 reductions and PCG scalars) on the same slices of a
+ * resident set as the FP64 entry points; cubic n in 16, 24, 32, 48, 64, 96,
+ * 128 only. One refinement step is: pgj_resident_certify (FP64 r = f - K u,
+ * |r|), pgj_inner_begin, pgj_inner_init, pgj_inner_iterate until no inner
+ * load is active, pgj_inner_end. No reference counterpart: the reference's
+ * refinement precedent is solve_ir_batched
+ * (solver/src/runtime/fi/source/femlib/batched/ir.py:23-107), which refines
+ * with a generic inner solver. */
+size_t pgj_plan32_bytes(const pgj_grid* g);
+/* FP32 copy of a plan (twiddles and symbol rounded from the FP64 plan). */
+int pgj_plan32_init(const pgj_grid* g, const void* plan, void* plan32, void* stream);
+/* y = (float) x for n values (fields, scales; a complex spectrum is 2n reals). */
+int pgj_to_f32(int64_t n, const double* x, float* y, void* stream);
+/* Workspace of the inner solver for a set of L loads (FP32 spectrum). */
+size_t pgj_inner_ws_bytes(const pgj_grid* g, int L);
+/* Per load of [l0, l0+nl): s_l = |r_l| from the FP64 state of the last
+ * certification, or 0 if that load is certified; r32 = r64 / s_l; u32 = 0;
+ * inner target |r32| <= rtol. state32 is a resident state block of L loads. */
+int pgj_inner_begin(const pgj_grid* g, int L, int l0, int nl, const double* r64, const void* state64, double rtol,
+                    float* r32, float* u32, void* state32, void* ws32, void* stream);
+/* pgj_resident_init / _stage / _iterate in FP32 (plan32, FP32 fields and vectors). */
+int pgj_inner_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32, const float* s,
+                   const void* s_hat, const float* r, float* z, void* state, void* ws, void* stream);
+int pgj_inner_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32, const float* a,
+                    const float* s, const void* s_hat, float* u, float* r, float* pold, float* pnew, float* q, float* z,
+                    void* state, void* ws, int stage, void* stream);
+int pgj_inner_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32, const float* a,
+                      const float* s, const void* s_hat, float* u, float* r, float* p0, float* p1, float* q, float* z,
+                      void* state, void* ws, int iters, void* stream);
+/* u64 += s_l * u32 for the loads of [l0, l0+nl) (FP64 accumulation). */
+int pgj_inner_end(const pgj_grid* g, int L, int l0, int nl, const float* u32, const void* ws32, double* u64,
+                  void* stream);
+
 /* ------------------------------------------------------------ OC updates */
 
 /* Fields are (n, n, n) FP64. `ws` holds pgj_oc_ws_bytes(n) bytes of per-block

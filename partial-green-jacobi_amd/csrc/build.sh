@@ -7,9 +7,9 @@ JOBS=${JOBS:-4}
 FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Xclang -target-feature -Xclang -packed-fp32-ops"
 TMP=$(mktemp -d)
 trap 'rm -rf "$TMP"' EXIT
-for src in common hex8 fft fft_static resident oc pcg; do
+for src in common hex8 fft fft_static resident oc pcg mixed; do
   hipcc $FLAGS -c "$HERE/$src.hip" -o "$TMP/$src.o" &
 done
 wait
-hipcc --offload-arch=gfx950 -shared -o "$OUT.tmp" "$TMP"/common.o "$TMP"/hex8.o "$TMP"/fft.o "$TMP"/fft_static.o "$TMP"/resident.o "$TMP"/oc.o "$TMP"/pcg.o
+hipcc --offload-arch=gfx950 -shared -o "$OUT.tmp" "$TMP"/common.o "$TMP"/hex8.o "$TMP"/fft.o "$TMP"/fft_static.o "$TMP"/resident.o "$TMP"/oc.o "$TMP"/pcg.o "$TMP"/mixed.o
 mv "$OUT.tmp" "$OUT"

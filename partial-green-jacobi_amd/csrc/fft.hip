@@ -500,8 +500,13 @@ int allow_lds(K kernel, size_t bytes) {
   return PGJ_OK;
 }
 
+#define PGJ_RUNTIME_FFT(A)                                                                              \
+  PGJ_REQUIRE((A).f.runtime_ok, "grid (%d,%d,%d) has only the compile-time resident FFT path", (A).nz, (A).ny, \
+              (A).nx)
+
 template <int MODE>
 int launch_fxy(FFTArgs& A, int L, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A);
   const FFTGeom& f = A.f;
   dim3 grid(A.nz, f.comps == 3 ? 1 : 3, L);
   if (int e = allow_lds(fxy_kernel<MODE>, f.lds_xy)) return e;
@@ -512,6 +517,7 @@ int launch_fxy(FFTArgs& A, int L, hipStream_t s) {
 
 template <int MODE>
 int launch_fz(FFTArgs& A, int L, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A);
   const FFTGeom& f = A.f;
   dim3 grid(f.nkb, A.ny, L);
   hipLaunchKernelGGL(fz_kernel<MODE>, grid, dim3(f.threads_z), f.lds_z, s, A);
@@ -521,6 +527,7 @@ int launch_fz(FFTArgs& A, int L, hipStream_t s) {
 
 template <int MODE>
 int launch_fxyi(FFTArgs& A, int L, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A);
   const FFTGeom& f = A.f;
   dim3 grid(A.nz, f.comps == 3 ? 1 : 3, L);
   if (int e = allow_lds(fxyi_kernel<MODE>, f.lds_xy)) return e;
@@ -529,12 +536,7 @@ int launch_fxyi(FFTArgs& A, int L, hipStream_t s) {
   return PGJ_OK;
 }
 
-#define PGJ_RUNTIME_FFT(A)                                                                              \
-  PGJ_REQUIRE((A).f.runtime_ok, "grid (%d,%d,%d) has only the compile-time resident FFT path", (A).nz, (A).ny, \
-              (A).nx)
-
 int launch_fxy_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
-  PGJ_RUNTIME_FFT(A);
   switch (mode) {
     case kFxyScale: return launch_fxy<kFxyScale>(A, L, s);
     case kFxyPre: return launch_fxy<kFxyPre>(A, L, s);
@@ -543,7 +545,6 @@ int launch_fxy_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
 }
 
 int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
-  PGJ_RUNTIME_FFT(A);
   switch (mode) {
     case kFzScale: return launch_fz<kFzScale>(A, L, s);
     case kFzPre: return launch_fz<kFzPre>(A, L, s);
@@ -553,7 +554,6 @@ int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
 }
 
 int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s) {
-  PGJ_RUNTIME_FFT(A);
   return mode == kFxyiPre ? launch_fxyi<kFxyiPre>(A, L, s) : launch_fxyi<kFxyiPCG>(A, L, s);
 }
 

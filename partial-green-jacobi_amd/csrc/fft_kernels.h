@@ -11,23 +11,28 @@ enum { kFxyScale = 0, kFxyPre = 1, kFxyPCG = 2 };
 enum { kFzScale = 0, kFzPre = 1, kFzInit = 2, kFzPCG = 3 };
 enum { kFxyiPre = 0, kFxyiPCG = 1 };
 
-struct FFTArgs {
+// Arguments of the FFT/Green kernels; T is the vector precision (FP64 plain
+// path, FP32 inner solver of the mixed-precision form). Partial sums and the
+// PCG scalars stay FP64 in both.
+template <typename T>
+struct FFTArgsT {
+  using C = cplx<T>;
   int nz, ny, nx;
   int64_t N;
   FFTGeom f;
-  const double2* twx;
-  const double2* twy;
-  const double2* twz;
-  const double* sym;
-  const double* rin;     // fxy pre: input residual
-  double* u;             // fxy pcg
-  double* r;             // fxy pcg
-  const double* p;       // fxy pcg
-  const double* q;       // fxy pcg
-  const double* s;       // scale (nullable)
-  const double2* shat;   // spectrum of s (nullable)
-  double2* spec;         // spectrum buffer (L x 3 x nf)
-  double* zout;          // fxyi output
+  const C* twx;
+  const C* twy;
+  const C* twz;
+  const T* sym;
+  const T* rin;          // fxy pre: input residual
+  T* u;                  // fxy pcg
+  T* r;                  // fxy pcg
+  const T* p;            // fxy pcg
+  const T* q;            // fxy pcg
+  const T* s;            // scale (nullable)
+  const C* shat;         // spectrum of s (nullable)
+  C* spec;               // spectrum buffer (L x 3 x nf)
+  T* zout;               // fxyi output
   double* pr;            // fxy partials  [l][z][c][2] = (sum r^2, sum r)
   double* pg;            // fz partials   [l][blk][4]  = (v.w, s.w_x, s.w_y, s.w_z)
   const double* ppq;     // hex8 partials [l][blk], ppq_count blocks in rows of ppq_stride
@@ -36,6 +41,7 @@ struct FFTArgs {
   int soa;               // vectors (and s) are SoA (L, 3, N) instead of AoS (L, N, 3)
   int group;             // loads per design (s is (designs, 3N), s_hat (designs, 3, nf)); 0: shared
 };
+using FFTArgs = FFTArgsT<double>;
 
 // Index of component c of node `node` of load l in a vector (and of s with l = 0).
 __device__ __forceinline__ int64_t vindex(const FFTArgs& A, int l, int c, int64_t node) {
@@ -49,8 +55,8 @@ int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 // Compile-time-shaped SoA kernels for cubic grids (n in 16, 24, 32, 48, 64, 96, 128).
 bool static_fft(const pgj_grid& g);
-int launch_fxy_static(FFTArgs& A, int L, int mode, hipStream_t s);
-int launch_fz_static(FFTArgs& A, int L, int mode, hipStream_t s);
-int launch_fxyi_static(FFTArgs& A, int L, hipStream_t s);
+template <typename T> int launch_fxy_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
+template <typename T> int launch_fz_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
+template <typename T> int launch_fxyi_static(FFTArgsT<T>& A, int L, hipStream_t s);
 
 }  // namespace pgj

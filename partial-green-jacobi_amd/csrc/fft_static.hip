@@ -22,19 +22,22 @@ struct Cube {
   static constexpr int KC = N == 16 ? 9 : N == 24 ? 13 : N == 32 ? 17 : N == 48 ? 5 : N == 64 ? 11 : N == 96 ? 7 : 5;
   static constexpr int NKB = NKX / KC;
   static_assert(NKX % KC == 0, "kx chunking must divide the half spectrum");
-  static constexpr size_t LDS_XY = size_t(N) * RS * 16 + 4 * 8 * 8;
-  static constexpr size_t LDS_Z = size_t(3) * N * KC * 16 + 4 * 8 * 8;
+  // LDS bytes for complex values of ES bytes, plus a reduction tail of 4 values x 8 waves
+  static constexpr size_t lds_xy(size_t es) { return size_t(N) * RS * es + 4 * 8 * 8; }
+  static constexpr size_t lds_z(size_t es) { return size_t(3) * N * KC * es + 4 * 8 * 8; }
   using Rows = Lines<N, N, 0, RS, 1, false>;          // x transforms (length M) on the N rows
   using Cols = Lines<NKX, NKX, 0, 1, RS, true>;       // y transforms (length N) on the NKX columns
   using ZCols = Lines<3 * KC, KC, N * KC, 1, KC, true>;  // z transforms on (component, column)
 };
 
 // ------------------------------------------------------------ forward xy
-template <int N, int MODE>
-__global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgs A) {
+template <int N, int MODE, typename R>
+__global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
   using C = Cube<N>;
+  using X = cplx<R>;
   constexpr int T = C::TXY;
-  extern __shared__ double2 plane[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  X* plane = reinterpret_cast<X*>(smem);
   double* red = reinterpret_cast<double*>(plane + N * C::RS);
   const int z = blockIdx.x, c = blockIdx.y, l = blockIdx.z;
   const int64_t NN = int64_t(N) * N * N;
@@ -55,33 +58,34 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgs A) {
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   double sq = 0.0, sm = 0.0;
   // two nodes (x even, x+1) per step: one packed complex of the x transform
+  const R ra = R(alpha);
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
-    double2 v;
+    X v;
     if (MODE == kFxyScale) {
-      v = *reinterpret_cast<const double2*>(A.s + soff + e);
+      v = *reinterpret_cast<const X*>(A.s + soff + e);
     } else if (MODE == kFxyPre) {
-      const double2 rv = *reinterpret_cast<const double2*>(A.rin + off + e);
-      sq = fma(rv.x, rv.x, fma(rv.y, rv.y, sq));
-      sm += rv.x + rv.y;
+      const X rv = *reinterpret_cast<const X*>(A.rin + off + e);
+      sq = fma(double(rv.x), double(rv.x), fma(double(rv.y), double(rv.y), sq));
+      sm += double(rv.x) + double(rv.y);
       if (A.s) {
-        const double2 sv = *reinterpret_cast<const double2*>(A.s + soff + e);
+        const X sv = *reinterpret_cast<const X*>(A.s + soff + e);
         v = c2(sv.x * rv.x, sv.y * rv.y);
       } else {
         v = rv;
       }
     } else {
-      const double2 uu = *reinterpret_cast<const double2*>(A.u + off + e);
-      const double2 rr = *reinterpret_cast<const double2*>(A.r + off + e);
-      const double2 pp = *reinterpret_cast<const double2*>(A.p + off + e);
-      const double2 qq = *reinterpret_cast<const double2*>(A.q + off + e);
-      const double2 un = c2(fma(alpha, pp.x, uu.x), fma(alpha, pp.y, uu.y));
-      const double2 rn = c2(fma(-alpha, qq.x, rr.x), fma(-alpha, qq.y, rr.y));
-      *reinterpret_cast<double2*>(A.u + off + e) = un;
-      *reinterpret_cast<double2*>(A.r + off + e) = rn;
-      sq = fma(rn.x, rn.x, fma(rn.y, rn.y, sq));
-      sm += rn.x + rn.y;
+      const X uu = *reinterpret_cast<const X*>(A.u + off + e);
+      const X rr = *reinterpret_cast<const X*>(A.r + off + e);
+      const X pp = *reinterpret_cast<const X*>(A.p + off + e);
+      const X qq = *reinterpret_cast<const X*>(A.q + off + e);
+      const X un = c2(fma(ra, pp.x, uu.x), fma(ra, pp.y, uu.y));
+      const X rn = c2(fma(-ra, qq.x, rr.x), fma(-ra, qq.y, rr.y));
+      *reinterpret_cast<X*>(A.u + off + e) = un;
+      *reinterpret_cast<X*>(A.r + off + e) = rn;
+      sq = fma(double(rn.x), double(rn.x), fma(double(rn.y), double(rn.y), sq));
+      sm += double(rn.x) + double(rn.y);
       if (A.s) {
-        const double2 sv = *reinterpret_cast<const double2*>(A.s + soff + e);
+        const X sv = *reinterpret_cast<const X*>(A.s + soff + e);
         v = c2(sv.x * rn.x, sv.y * rn.y);
       } else {
         v = rn;
@@ -105,26 +109,28 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgs A) {
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
   for (int b = threadIdx.x; b < PAIRS; b += T) {
     const int row = b / (HALF + 1), k = b % (HALF + 1);
-    double2* L = plane + row * C::RS;
-    const double2 zk = L[k % C::M], zm = L[(C::M - k) % C::M];
-    const double2 E = c2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-    const double2 O = c2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
-    const double2 wo = cmul(A.twx[k], O);
+    X* L = plane + row * C::RS;
+    const X zk = L[k % C::M], zm = L[(C::M - k) % C::M];
+    const X E = c2(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
+    const X O = c2(R(0.5) * (zk.y + zm.y), R(-0.5) * (zk.x - zm.x));
+    const X wo = cmul(A.twx[k], O);
     L[k] = cadd(E, wo);
     if (C::M - k != k) L[C::M - k] = conjd(csub(E, wo));
   }
   __syncthreads();
   sfft<false, N, typename C::Cols, T>(plane, A.twy, 1);
-  double2* out = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
+  X* out = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
   for (int e = threadIdx.x; e < N * C::NKX; e += T) out[e] = plane[e];
 }
 
 // ---------------------------------------------------------------- z + Green
-template <int N, int MODE>
-__global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
+template <int N, int MODE, typename R>
+__global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
   using C = Cube<N>;
+  using X = cplx<R>;
   constexpr int T = C::TZ, KC = C::KC, NKX = C::NKX;
-  extern __shared__ double2 cols[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  X* cols = reinterpret_cast<X*>(smem);
   double* red = reinterpret_cast<double*>(cols + 3 * N * KC);
   // ky is the fastest grid index: the kx chunks of one ky row (which share
   // cache lines) get block ids N apart, a multiple of 8 for every supported N,
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
     for (int c = 0; c < 3; ++c)
       mean[c] = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
   }
-  double2* S = A.spec + int64_t(l) * 3 * C::NF;
+  X* S = A.spec + int64_t(l) * 3 * C::NF;
   for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
     const int kcol = e % KC, cz = e / KC;  // cz = c * N + zz
     cols[e] = S[(int64_t(cz) * N + y) * NKX + kx0 + kcol];
@@ -191,30 +197,32 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
     const int kcol = e % KC, zz = e / KC;
     const int kx = kx0 + kcol;
     const int64_t fidx = (int64_t(zz) * N + y) * NKX + kx;
-    double2 v[3], sh[3];
+    X v[3], sh[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = cols[(c * N + zz) * KC + kcol];
     if (A.shat) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
+        const R mc = R(mean[c]);
         sh[c] = A.shat[(int64_t(design_of(l, A.group)) * 3 + c) * C::NF + fidx];
-        v[c] = c2(fma(-mean[c], sh[c].x, v[c].x), fma(-mean[c], sh[c].y, v[c].y));
+        v[c] = c2(fma(-mc, sh[c].x, v[c].x), fma(-mc, sh[c].y, v[c].y));
       }
     }
-    const double g0 = A.sym[fidx], g1 = A.sym[C::NF + fidx], g2 = A.sym[2 * C::NF + fidx];
-    const double g3 = A.sym[3 * C::NF + fidx], g4 = A.sym[4 * C::NF + fidx], g5 = A.sym[5 * C::NF + fidx];
-    double2 w[3];
+    const R g0 = A.sym[fidx], g1 = A.sym[C::NF + fidx], g2 = A.sym[2 * C::NF + fidx];
+    const R g3 = A.sym[3 * C::NF + fidx], g4 = A.sym[4 * C::NF + fidx], g5 = A.sym[5 * C::NF + fidx];
+    X w[3];
     w[0] = c2(g0 * v[0].x + g3 * v[1].x + g4 * v[2].x, g0 * v[0].y + g3 * v[1].y + g4 * v[2].y);
     w[1] = c2(g3 * v[0].x + g1 * v[1].x + g5 * v[2].x, g3 * v[0].y + g1 * v[1].y + g5 * v[2].y);
     w[2] = c2(g4 * v[0].x + g5 * v[1].x + g2 * v[2].x, g4 * v[0].y + g5 * v[1].y + g2 * v[2].y);
     const double wgt = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
     double t = 0.0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) t += v[c].x * w[c].x + v[c].y * w[c].y;
+    for (int c = 0; c < 3; ++c) t += double(v[c].x) * double(w[c].x) + double(v[c].y) * double(w[c].y);
     rz = fma(wgt, t, rz);
     if (A.shat) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sw[c] = fma(wgt, sh[c].x * w[c].x + sh[c].y * w[c].y, sw[c]);
+      for (int c = 0; c < 3; ++c)
+        sw[c] = fma(wgt, double(sh[c].x) * double(w[c].x) + double(sh[c].y) * double(w[c].y), sw[c]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) cols[(c * N + zz) * KC + kcol] = w[c];
@@ -237,11 +245,13 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgs A) {
 }
 
 // --------------------------------------------------------------- inverse xy
-template <int N>
-__global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgs A) {
+template <int N, typename R>
+__global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
   using C = Cube<N>;
+  using X = cplx<R>;
   constexpr int T = C::TXY;
-  extern __shared__ double2 plane[];
+  extern __shared__ __align__(16) unsigned char smem[];
+  X* plane = reinterpret_cast<X*>(smem);
   const int z = blockIdx.x, c = blockIdx.y, l = blockIdx.z;
   const int64_t NN = int64_t(N) * N * N;
   if (threadIdx.x == 0 && z == 0 && c == 0) {
@@ -252,27 +262,27 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgs A) {
   if (!A.st.active_next[l]) return;
   double mean = 0.0;
   if (A.shat) mean = wave_reduce_partials(A.pg + int64_t(l) * C::NKB * N * 4 + 1 + c, C::NKB * N, 4) / double(NN);
-  const double2* in = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
+  const X* in = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
   for (int e = threadIdx.x; e < N * C::NKX; e += T) plane[e] = in[e];
   __syncthreads();
   sfft<true, N, typename C::Cols, T>(plane, A.twy, 1);
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
   for (int b = threadIdx.x; b < PAIRS; b += T) {
     const int row = b / (HALF + 1), k = b % (HALF + 1);
-    double2* L = plane + row * C::RS;
-    double2 xk = L[k], xm = L[C::M - k];
+    X* L = plane + row * C::RS;
+    X xk = L[k], xm = L[C::M - k];
     if (k == 0) {
-      xk.y = 0.0;
-      xm.y = 0.0;
+      xk.y = R(0);
+      xm.y = R(0);
     }
-    const double2 wk = conjd(A.twx[k]);
-    const double2 e1 = c2(xk.x + xm.x, xk.y - xm.y);
-    const double2 o1 = cmul(c2(xk.x - xm.x, xk.y + xm.y), wk);
-    const double2 zk = c2(e1.x - o1.y, e1.y + o1.x);
+    const X wk = conjd(A.twx[k]);
+    const X e1 = c2(xk.x + xm.x, xk.y - xm.y);
+    const X o1 = cmul(c2(xk.x - xm.x, xk.y + xm.y), wk);
+    const X zk = c2(e1.x - o1.y, e1.y + o1.x);
     if (k != 0 && C::M - k != k) {
-      const double2 wm = conjd(A.twx[C::M - k]);
-      const double2 e2 = c2(xm.x + xk.x, xm.y - xk.y);
-      const double2 o2 = cmul(c2(xm.x - xk.x, xm.y + xk.y), wm);
+      const X wm = conjd(A.twx[C::M - k]);
+      const X e2 = c2(xm.x + xk.x, xm.y - xk.y);
+      const X o2 = cmul(c2(xm.x - xk.x, xm.y + xk.y), wm);
       L[C::M - k] = c2(e2.x - o2.y, e2.y + o2.x);
     }
     L[k] = zk;
@@ -283,13 +293,14 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgs A) {
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
     const int x = e % N, y = e / N;
-    const double2 w = plane[y * C::RS + x / 2];
-    double2 v = w;
+    const X w = plane[y * C::RS + x / 2];
+    X v = w;
     if (A.s) {
-      const double2 sv = *reinterpret_cast<const double2*>(A.s + soff + e);
-      v = c2(sv.x * w.x - mean, sv.y * w.y - mean);
+      const X sv = *reinterpret_cast<const X*>(A.s + soff + e);
+      const R m = R(mean);
+      v = c2(sv.x * w.x - m, sv.y * w.y - m);
     }
-    *reinterpret_cast<double2*>(A.zout + off + e) = v;
+    *reinterpret_cast<X*>(A.zout + off + e) = v;
   }
 }
 
@@ -306,99 +317,93 @@ int allow_lds_static(K kernel, size_t bytes) {
   return PGJ_OK;
 }
 
-template <int N>
-int fxy_static_n(FFTArgs& A, int L, int mode, hipStream_t s) {
+template <int N, typename R>
+int fxy_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   using C = Cube<N>;
+  const size_t lds = C::lds_xy(sizeof(cplx<R>));
   dim3 grid(N, 3, L), block(C::TXY);
   switch (mode) {
     case kFxyScale:
-      if (int e = allow_lds_static(fxy_static<N, kFxyScale>, C::LDS_XY)) return e;
-      hipLaunchKernelGGL((fxy_static<N, kFxyScale>), grid, block, C::LDS_XY, s, A);
+      if (int e = allow_lds_static(fxy_static<N, kFxyScale, R>, lds)) return e;
+      hipLaunchKernelGGL((fxy_static<N, kFxyScale, R>), grid, block, lds, s, A);
       break;
     case kFxyPre:
-      if (int e = allow_lds_static(fxy_static<N, kFxyPre>, C::LDS_XY)) return e;
-      hipLaunchKernelGGL((fxy_static<N, kFxyPre>), grid, block, C::LDS_XY, s, A);
+      if (int e = allow_lds_static(fxy_static<N, kFxyPre, R>, lds)) return e;
+      hipLaunchKernelGGL((fxy_static<N, kFxyPre, R>), grid, block, lds, s, A);
       break;
     default:
-      if (int e = allow_lds_static(fxy_static<N, kFxyPCG>, C::LDS_XY)) return e;
-      hipLaunchKernelGGL((fxy_static<N, kFxyPCG>), grid, block, C::LDS_XY, s, A);
+      if (int e = allow_lds_static(fxy_static<N, kFxyPCG, R>, lds)) return e;
+      hipLaunchKernelGGL((fxy_static<N, kFxyPCG, R>), grid, block, lds, s, A);
   }
   PGJ_LAUNCHED("fxy_static");
   return PGJ_OK;
 }
 
-template <int N>
-int fz_static_n(FFTArgs& A, int L, int mode, hipStream_t s) {
+template <int N, typename R>
+int fz_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   using C = Cube<N>;
+  const size_t lds = C::lds_z(sizeof(cplx<R>));
   dim3 grid(N, C::NKB, L), block(C::TZ);
   switch (mode) {
-    case kFzScale: hipLaunchKernelGGL((fz_static<N, kFzScale>), grid, block, C::LDS_Z, s, A); break;
-    case kFzPre: hipLaunchKernelGGL((fz_static<N, kFzPre>), grid, block, C::LDS_Z, s, A); break;
-    case kFzInit: hipLaunchKernelGGL((fz_static<N, kFzInit>), grid, block, C::LDS_Z, s, A); break;
-    default: hipLaunchKernelGGL((fz_static<N, kFzPCG>), grid, block, C::LDS_Z, s, A);
+    case kFzScale: hipLaunchKernelGGL((fz_static<N, kFzScale, R>), grid, block, lds, s, A); break;
+    case kFzPre: hipLaunchKernelGGL((fz_static<N, kFzPre, R>), grid, block, lds, s, A); break;
+    case kFzInit: hipLaunchKernelGGL((fz_static<N, kFzInit, R>), grid, block, lds, s, A); break;
+    default: hipLaunchKernelGGL((fz_static<N, kFzPCG, R>), grid, block, lds, s, A);
   }
   PGJ_LAUNCHED("fz_static");
   return PGJ_OK;
 }
 
-template <int N>
-int fxyi_static_n(FFTArgs& A, int L, hipStream_t s) {
+template <int N, typename R>
+int fxyi_static_n(FFTArgsT<R>& A, int L, hipStream_t s) {
   using C = Cube<N>;
-  if (int e = allow_lds_static(fxyi_static<N>, C::LDS_XY)) return e;
-  hipLaunchKernelGGL(fxyi_static<N>, dim3(N, 3, L), dim3(C::TXY), C::LDS_XY, s, A);
+  const size_t lds = C::lds_xy(sizeof(cplx<R>));
+  if (int e = allow_lds_static(fxyi_static<N, R>, lds)) return e;
+  hipLaunchKernelGGL((fxyi_static<N, R>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);
   PGJ_LAUNCHED("fxyi_static");
   return PGJ_OK;
 }
 
 bool static_fft(const pgj_grid& g) { return static_cube(g); }
 
-template <int N> struct FxyCall { static int go(FFTArgs& A, int L, int m, hipStream_t s) { return fxy_static_n<N>(A, L, m, s); } };
-template <int N> struct FzCall { static int go(FFTArgs& A, int L, int m, hipStream_t s) { return fz_static_n<N>(A, L, m, s); } };
-template <int N> struct FxyiCall { static int go(FFTArgs& A, int L, hipStream_t s) { return fxyi_static_n<N>(A, L, s); } };
-
-int launch_fxy_static(FFTArgs& A, int L, int mode, hipStream_t s) {
-#define CALL(n) FxyCall<n>::go(A, L, mode, s)
-  switch (A.nx) {
-    case 16: return CALL(16);
-    case 24: return CALL(24);
-    case 32: return CALL(32);
-    case 48: return CALL(48);
-    case 64: return CALL(64);
-    case 96: return CALL(96);
-    case 128: return CALL(128);
-    default: set_error("no static FFT for n=%d", A.nx); return PGJ_ERR_UNSUPPORTED;
+#define PGJ_STATIC_SWITCH(CALL)                                       \
+  switch (A.nx) {                                                     \
+    case 16: return CALL(16);                                         \
+    case 24: return CALL(24);                                         \
+    case 32: return CALL(32);                                         \
+    case 48: return CALL(48);                                         \
+    case 64: return CALL(64);                                         \
+    case 96: return CALL(96);                                         \
+    case 128: return CALL(128);                                       \
+    default: set_error("no static FFT for n=%d", A.nx); return PGJ_ERR_UNSUPPORTED; \
   }
+
+template <typename R>
+int launch_fxy_static(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
+#define CALL(n) fxy_static_n<n, R>(A, L, mode, s)
+  PGJ_STATIC_SWITCH(CALL)
 #undef CALL
 }
 
-int launch_fz_static(FFTArgs& A, int L, int mode, hipStream_t s) {
-#define CALL(n) FzCall<n>::go(A, L, mode, s)
-  switch (A.nx) {
-    case 16: return CALL(16);
-    case 24: return CALL(24);
-    case 32: return CALL(32);
-    case 48: return CALL(48);
-    case 64: return CALL(64);
-    case 96: return CALL(96);
-    case 128: return CALL(128);
-    default: set_error("no static FFT for n=%d", A.nx); return PGJ_ERR_UNSUPPORTED;
-  }
+template <typename R>
+int launch_fz_static(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
+#define CALL(n) fz_static_n<n, R>(A, L, mode, s)
+  PGJ_STATIC_SWITCH(CALL)
 #undef CALL
 }
 
-int launch_fxyi_static(FFTArgs& A, int L, hipStream_t s) {
-#define CALL(n) FxyiCall<n>::go(A, L, s)
-  switch (A.nx) {
-    case 16: return CALL(16);
-    case 24: return CALL(24);
-    case 32: return CALL(32);
-    case 48: return CALL(48);
-    case 64: return CALL(64);
-    case 96: return CALL(96);
-    case 128: return CALL(128);
-    default: set_error("no static FFT for n=%d", A.nx); return PGJ_ERR_UNSUPPORTED;
-  }
+template <typename R>
+int launch_fxyi_static(FFTArgsT<R>& A, int L, hipStream_t s) {
+#define CALL(n) fxyi_static_n<n, R>(A, L, s)
+  PGJ_STATIC_SWITCH(CALL)
 #undef CALL
 }
+
+template int launch_fxy_static<double>(FFTArgsT<double>&, int, int, hipStream_t);
+template int launch_fz_static<double>(FFTArgsT<double>&, int, int, hipStream_t);
+template int launch_fxyi_static<double>(FFTArgsT<double>&, int, hipStream_t);
+template int launch_fxy_static<float>(FFTArgsT<float>&, int, int, hipStream_t);
+template int launch_fz_static<float>(FFTArgsT<float>&, int, int, hipStream_t);
+template int launch_fxyi_static<float>(FFTArgsT<float>&, int, hipStream_t);
 
 }  // namespace pgj

@@ -9,10 +9,10 @@ namespace pgj {
 
 // Raw inputs of one node row segment: the vector itself (apply) or the pair
 // (z, p_old) the fused update p = z + beta p_old is formed from (PCG).
-template <int MODE, int R, int XPL>
+template <int MODE, int R, int XPL, typename T>
 struct Rows {
-  double v[R + 1][XPL][3];
-  double w[MODE == kModePCG ? R + 1 : 1][XPL][3];
+  T v[R + 1][XPL][3];
+  T w[MODE == kModePCG ? R + 1 : 1][XPL][3];
 };
 
 // Row-march HEX8 apply. Lanes map to x (XPL consecutive x per lane), each of
@@ -20,16 +20,18 @@ struct Rows {
 // halo wave computes the element row below the block, and contributions to a
 // neighbour wave's first node row go through a double-buffered LDS slot. The
 // next plane's inputs are fetched before the current layer is computed.
-template <int MODE, int XPL, int R, bool CUBIC, int MAXT>
-__global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
+template <int MODE, int XPL, int R, bool CUBIC, int MAXT, typename T>
+__global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   // the drop-in apply works on AoS vectors, the resident iteration on SoA
   constexpr bool SOA = MODE == kModePCG;
-  extern __shared__ double lds[];
+  constexpr unsigned ES = sizeof(T);
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = reinterpret_cast<T*>(smem);
   const int lane = threadIdx.x & 63;
   // wave index and everything derived from it are wave-uniform: keep them in SGPRs
   const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int l = blockIdx.z;
-  const double* __restrict__ mat = A.a + size_t(design_of(l, A.group)) * A.N;
+  const T* __restrict__ mat = A.a + size_t(design_of(l, A.group)) * A.N;
   const RowMarch& c = A.cfg;
   const int W = c.W;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
@@ -51,13 +53,13 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
     }
   } else if (A.mask && !A.mask[l]) {
     // Inactive load: zeros, as the reference's pre-zeroed output.
-    double* o = A.out + size_t(l) * 3 * N;
+    T* o = A.out + size_t(l) * 3 * N;
     if (wave < W && lane_on) {
       for (int j = 0; j < R; ++j)
         for (int zi = 0; zi < c.ZC; ++zi)
           for (int xi = 0; xi < XPL; ++xi) {
             const unsigned node = unsigned((z0 + zi) * ny + (y0 + wave * R + j)) * unsigned(nx) + xo + xi;
-            for (int d = 0; d < 3; ++d) o[3 * node + d] = 0.0;
+            for (int d = 0; d < 3; ++d) o[3 * node + d] = T(0);
           }
     }
     return;
@@ -67,21 +69,21 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
   const int ybase = halo ? y0 - 1 : y0 + wave * R;
   const int rows = halo ? 1 : R;  // element rows computed by this wave
   // uniform base pointers per component (SoA) or per load (AoS)
-  const double* __restrict__ in = A.u + size_t(l) * 3 * N;
-  const double* __restrict__ old = SOA ? A.pold + size_t(l) * 3 * N : nullptr;
-  double* __restrict__ pn = SOA ? A.pnew + size_t(l) * 3 * N : nullptr;
-  double* __restrict__ out = A.out + size_t(l) * 3 * N;
+  const T* __restrict__ in = A.u + size_t(l) * 3 * N;
+  const T* __restrict__ old = SOA ? A.pold + size_t(l) * 3 * N : nullptr;
+  T* __restrict__ pn = SOA ? A.pnew + size_t(l) * 3 * N : nullptr;
+  T* __restrict__ out = A.out + size_t(l) * 3 * N;
   // element offsets in bytes (32-bit): uniform base + 32-bit offset selects the SGPR-base addressing form
-  auto at = [&](unsigned node, int d) -> unsigned { return 8u * (SOA ? unsigned(d) * N + node : 3u * node + unsigned(d)); };
-  auto ld = [](const double* base, unsigned off) -> double {
-    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+  auto at = [&](unsigned node, int d) -> unsigned { return ES * (SOA ? unsigned(d) * N + node : 3u * node + unsigned(d)); };
+  auto ld = [](const T* base, unsigned off) -> T {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
   };
-  auto st = [](double* base, unsigned off, double v) { *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off) = v; };
+  auto st = [](T* base, unsigned off, T v) { *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v; };
   auto row_of = [&](int plane, int j) -> unsigned {
     return unsigned(plane * ny + wrapi(ybase + j, ny)) * unsigned(nx);
   };
 
-  auto fetch = [&](int plane, Rows<MODE, R, XPL>& dst, double (&arow)[R][XPL]) {
+  auto fetch = [&](int plane, Rows<MODE, R, XPL, T>& dst, T (&arow)[R][XPL]) {
 #pragma unroll
     for (int j = 0; j <= R; ++j) {
       if (j > rows) break;
@@ -95,12 +97,13 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
           // p_old is read unconditionally (ignored right after init) so the load keeps its SGPR base
           if constexpr (MODE == kModePCG) dst.w[j][xi][d] = ld(old, at(node, d));
         }
-        if (j < R) arow[j][xi] = ld(mat, 8u * node);
+        if (j < R) arow[j][xi] = ld(mat, ES * node);
       }
     }
   };
   // Node values of a fetched plane; p_new is stored for owned rows.
-  auto settle = [&](int plane, const Rows<MODE, R, XPL>& src, double (&dst)[R + 1][XPL][3], bool write_p) {
+  const T tbeta = T(beta);
+  auto settle = [&](int plane, const Rows<MODE, R, XPL, T>& src, T (&dst)[R + 1][XPL][3], bool write_p) {
 #pragma unroll
     for (int j = 0; j <= R; ++j) {
       if (j > rows) break;
@@ -109,9 +112,9 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
       for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          double v = src.v[j][xi][d];
+          T v = src.v[j][xi][d];
           if constexpr (MODE == kModePCG) {
-            if (!first) v = fma(beta, src.w[j][xi][d], v);
+            if (!first) v = fma(tbeta, src.w[j][xi][d], v);
             if (write_p && j < rows && !halo && lane_on) st(pn, at(rb + xo + xi, d), v);
           }
           dst[j][xi][d] = v;
@@ -119,23 +122,23 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
     }
   };
 
-  double nodeL[R + 1][XPL][3], nodeU[R + 1][XPL][3];
-  double carry[R][XPL][3];
-  double accL[R + 1][XPL][3], accU[R + 1][XPL][3];
-  double aL[R][XPL], aU[R][XPL], aN[R][XPL];
+  T nodeL[R + 1][XPL][3], nodeU[R + 1][XPL][3];
+  T carry[R][XPL][3];
+  T accL[R + 1][XPL][3], accU[R + 1][XPL][3];
+  T aL[R][XPL], aU[R][XPL], aN[R][XPL];
   double pq = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j)
 #pragma unroll
     for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) carry[j][xi][d] = 0.0;
+      for (int d = 0; d < 3; ++d) carry[j][xi][d] = T(0);
 
   const int src_right = (xl + 1) % c.nl;       // lane holding x + XPL
   const int src_left = (xl + c.nl - 1) % c.nl;  // lane holding x - 1
   const int slot_size = (c.W + 1) * 64 * 6 * XPL;
 
-  Rows<MODE, R, XPL> raw;
+  Rows<MODE, R, XPL, T> raw;
   fetch(wrapi(z0 - 1, nz), raw, aL);
   settle(wrapi(z0 - 1, nz), raw, nodeL, false);
   fetch(z0, raw, aU);
@@ -149,14 +152,14 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
       for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          accL[j][xi][d] = (j < R) ? carry[j][xi][d] : 0.0;
-          accU[j][xi][d] = 0.0;
+          accL[j][xi][d] = (j < R) ? carry[j][xi][d] : T(0);
+          accU[j][xi][d] = T(0);
         }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       if (j >= rows) break;
       // Right neighbours (x + XPL) of the four node rows this element row touches.
-      double nb[2][2][3];
+      T nb[2][2][3];
 #pragma unroll
       for (int oy = 0; oy < 2; ++oy)
 #pragma unroll
@@ -164,16 +167,16 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
           nb[oy][0][d] = __shfl(nodeL[j + oy][0][d], src_right, 64);
           nb[oy][1][d] = __shfl(nodeU[j + oy][0][d], src_right, 64);
         }
-      double spill[2][2][3];  // contributions of this lane's last element to x + 1 (next lane)
+      T spill[2][2][3];  // contributions of this lane's last element to x + 1 (next lane)
 #pragma unroll
       for (int xi = 0; xi < XPL; ++xi) {
-        double ue[24], fe[24];
+        T ue[24], fe[24];
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
           const int ox = n & 1, oy = (n >> 1) & 1, oz = (n >> 2) & 1;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            double v;
+            T v;
             if (ox == 0 || xi + 1 < XPL) {
               const int xs = xi + ox;
               v = oz ? nodeU[j + oy][xs][d] : nodeL[j + oy][xs][d];
@@ -183,13 +186,13 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
             ue[3 * n + d] = v;
           }
         }
-        hex8_element<CUBIC>(ue, aL[j][xi], A.coef, fe);
+        hex8_element<CUBIC, T>(ue, aL[j][xi], A.coef, fe);
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
           const int ox = n & 1, oy = (n >> 1) & 1, oz = (n >> 2) & 1;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            const double v = fe[3 * n + d];
+            const T v = fe[3 * n + d];
             if (ox == 0) {
               if (oz) accU[j + oy][xi][d] += v; else accL[j + oy][xi][d] += v;
             } else if (xi + 1 < XPL) {
@@ -209,8 +212,8 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
         }
     }
     // Export the contributions to the first node row of the next wave.
-    double* slots = lds + (li & 1) * slot_size;
-    double* slot = slots + wave * 64 * 6 * XPL;
+    T* slots = lds + (li & 1) * slot_size;
+    T* slot = slots + wave * 64 * 6 * XPL;
 #pragma unroll
     for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
       }
     __syncthreads();
     if (!halo) {
-      const double* from = slots + (wave == 0 ? W : wave - 1) * 64 * 6 * XPL;
+      const T* from = slots + (wave == 0 ? W : wave - 1) * 64 * 6 * XPL;
 #pragma unroll
       for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
               st(out, at(rb + xo + xi, d), accL[j][xi][d]);
-              if (MODE == kModePCG) pq = fma(nodeL[j][xi][d], accL[j][xi][d], pq);
+              if (MODE == kModePCG) pq = fma(double(nodeL[j][xi][d]), double(accL[j][xi][d]), pq);
             }
           }
         }
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgs A) {
       }
   }
   if (MODE == kModePCG) {
-    const double total = block_sum(pq, lds + 2 * slot_size);
+    const double total = block_sum(pq, reinterpret_cast<double*>(lds + 2 * slot_size));
     if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
   }
 }
@@ -620,34 +623,37 @@ int check_grid(const pgj_grid* g) {
   return PGJ_OK;
 }
 
-template <int MODE, int XPL, int R, int MAXT>
-int launch_rowmarch_t(const pgj_grid& g, int L, RowArgs& A, hipStream_t s) {
+template <int MODE, int XPL, int R, int MAXT, typename T>
+int launch_rowmarch_t(const pgj_grid& g, int L, RowArgsT<T>& A, hipStream_t s) {
   const RowMarch& c = A.cfg;
   dim3 grid(c.by, c.bz, L), block(64 * (c.W + 1));
-  const size_t lds = (2 * size_t(c.W + 1) * 64 * 6 * XPL + 64) * 8;
+  const size_t lds = 2 * size_t(c.W + 1) * 64 * 6 * XPL * sizeof(T) + 64 * 8;
   if (cubic(g))
-    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true, MAXT>), grid, block, lds, s, A);
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true, MAXT, T>), grid, block, lds, s, A);
   else
-    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false, MAXT>), grid, block, lds, s, A);
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false, MAXT, T>), grid, block, lds, s, A);
   PGJ_LAUNCHED("hex8_rowmarch");
   return PGJ_OK;
 }
 
-template <int MODE>
-int launch_rowmarch_m(const pgj_grid& g, int L, RowArgs& A, hipStream_t s) {
+template <int MODE, typename T>
+int launch_rowmarch_m(const pgj_grid& g, int L, RowArgsT<T>& A, hipStream_t s) {
   const RowMarch& c = A.cfg;
   if (c.R == 2) return launch_rowmarch_t<MODE, 1, 2, 320>(g, L, A, s);
   if (c.xpl == 1) return launch_rowmarch_t<MODE, 1, 1, 576>(g, L, A, s);
   return launch_rowmarch_t<MODE, 2, 1, 320>(g, L, A, s);
 }
 
-int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgs& A, hipStream_t s) {
+template <typename T>
+int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgsT<T>& A, hipStream_t s) {
   A.nz = g.nz; A.ny = g.ny; A.nx = g.nx;
   A.N = int64_t(g.nx) * g.ny * g.nz;
   A.cfg = rowmarch_config(g, L);
   A.coef = coef_of(g);
   return mode == kModeApply ? launch_rowmarch_m<kModeApply>(g, L, A, s) : launch_rowmarch_m<kModePCG>(g, L, A, s);
 }
+template int launch_rowmarch<double>(const pgj_grid&, int, int, RowArgsT<double>&, hipStream_t);
+template int launch_rowmarch<float>(const pgj_grid&, int, int, RowArgsT<float>&, hipStream_t);
 
 int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_t s) {
   A.nz = g.nz; A.ny = g.ny; A.nx = g.nx;

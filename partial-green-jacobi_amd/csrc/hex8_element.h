@@ -32,10 +32,11 @@ struct ElementCoef {
   double vol;       // hx*hy*hz
 };
 
-// f = a * Ke(C0) * u for one element. u and f are 24 values, dof 3a+d.
-template <bool kCubic>
-PGJ_HD inline void hex8_element(const double (&u)[24], double a, const ElementCoef& c, double (&f)[24]) {
-  double U[3][8];
+// f = a * Ke(C0) * u for one element. u and f are 24 values, dof 3a+d; T is
+// the arithmetic precision (the coefficients are rounded to it).
+template <bool kCubic, typename T = double>
+PGJ_HD inline void hex8_element(const T (&u)[24], T a, const ElementCoef& c, T (&f)[24]) {
+  T U[3][8];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -48,39 +49,39 @@ PGJ_HD inline void hex8_element(const double (&u)[24], double a, const ElementCo
 #pragma unroll
       for (int n = 0; n < 8; ++n)
         if (!(n & b)) {
-          const double lo = U[d][n], hi = U[d][n | b];
+          const T lo = U[d][n], hi = U[d][n | b];
           U[d][n] = lo + hi;
           U[d][n | b] = hi - lo;
         }
   // Gradient scale: with a cubic voxel g^2 is folded into k_t.
-  const double gs0 = kCubic ? 1.0 : c.g[0];
-  const double gs1 = kCubic ? 1.0 : c.g[1];
-  const double gs2 = kCubic ? 1.0 : c.g[2];
-  const double gs[3] = {gs0, gs1, gs2};
-  const double base = kCubic ? a * c.vol * c.g[0] * c.g[0] : a * c.vol;
-  double F[3][8];
+  const T gs0 = kCubic ? T(1) : T(c.g[0]);
+  const T gs1 = kCubic ? T(1) : T(c.g[1]);
+  const T gs2 = kCubic ? T(1) : T(c.g[2]);
+  const T gs[3] = {gs0, gs1, gs2};
+  const T base = kCubic ? T(double(a) * c.vol * c.g[0] * c.g[0]) : T(double(a) * c.vol);
+  T F[3][8];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int n = 0; n < 8; ++n) F[d][n] = 0.0;
+    for (int n = 0; n < 8; ++n) F[d][n] = T(0);
 #pragma unroll
   for (int t = 0; t < 7; ++t) {
     const int bits = ((t & 1) ? 1 : 0) + ((t & 2) ? 1 : 0) + ((t & 4) ? 1 : 0);
-    const double w = bits == 0 ? 1.0 : (bits == 1 ? (1.0 / 3.0) : (1.0 / 9.0));
-    const double k = base * w;
+    const T w = bits == 0 ? T(1) : (bits == 1 ? T(1.0 / 3.0) : T(1.0 / 9.0));
+    const T k = base * w;
     // Gradient entries G[d][j] = g_j U_d[t | e_j], present iff bit j of t is clear.
-    double G[3][3];
+    T G[3][3];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) G[d][j] = (t & (1 << j)) ? 0.0 : gs[j] * U[d][t | (1 << j)];
-    double tr = 0.0;
+      for (int j = 0; j < 3; ++j) G[d][j] = (t & (1 << j)) ? T(0) : gs[j] * U[d][t | (1 << j)];
+    T tr = T(0);
 #pragma unroll
     for (int j = 0; j < 3; ++j)
       if (!(t & (1 << j))) tr += G[j][j];
-    const double ltr = c.lam * k * tr;
-    const double two_mu_k = 2.0 * c.mu * k;
-    const double mu_k = c.mu * k;
+    const T ltr = T(c.lam) * k * tr;
+    const T two_mu_k = T(2.0 * c.mu) * k;
+    const T mu_k = T(c.mu) * k;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
       if (!(t & (1 << j))) F[j][t | (1 << j)] += gs[j] * (ltr + two_mu_k * G[j][j]);
@@ -91,7 +92,7 @@ PGJ_HD inline void hex8_element(const double (&u)[24], double a, const ElementCo
         const bool jd = !(t & (1 << j));  // G[d][j] present
         const bool dj = !(t & (1 << d));  // G[j][d] present
         if (jd || dj) {
-          const double s = mu_k * ((jd ? G[d][j] : 0.0) + (dj ? G[j][d] : 0.0));
+          const T s = mu_k * ((jd ? G[d][j] : T(0)) + (dj ? G[j][d] : T(0)));
           if (jd) F[d][t | (1 << j)] += gs[j] * s;
           if (dj) F[j][t | (1 << d)] += gs[d] * s;
         }
@@ -105,7 +106,7 @@ PGJ_HD inline void hex8_element(const double (&u)[24], double a, const ElementCo
 #pragma unroll
       for (int n = 0; n < 8; ++n)
         if (!(n & b)) {
-          const double lo = F[d][n], hi = F[d][n | b];
+          const T lo = F[d][n], hi = F[d][n | b];
           F[d][n] = lo - hi;
           F[d][n | b] = lo + hi;
         }

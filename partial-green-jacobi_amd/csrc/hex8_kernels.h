@@ -9,18 +9,19 @@ namespace pgj {
 
 enum { kModeApply = 0, kModePCG = 1 };
 
-struct RowArgs {
+template <typename T>
+struct RowArgsT {
   int nz, ny, nx;
   int64_t N;
   RowMarch cfg;
   ElementCoef coef;
-  const double* a;
-  const double* u;       // apply: input vectors; PCG: preconditioned residual z
-  double* out;           // apply: K u; PCG: q = K p_new
+  const T* a;
+  const T* u;            // apply: input vectors; PCG: preconditioned residual z
+  T* out;                // apply: K u; PCG: q = K p_new
   const uint8_t* mask;   // apply: optional active mask
   // PCG fusion
-  const double* pold;
-  double* pnew;
+  const T* pold;
+  T* pnew;
   StateView st;
   const double* pg;      // FZ partials (rz part first of 4), per load stride pg_stride
   int pg_count, pg_stride;
@@ -28,6 +29,7 @@ struct RowArgs {
   int ppq_stride;
   int group;             // loads per design (a is (designs, N)); 0: one shared a
 };
+using RowArgs = RowArgsT<double>;
 
 // Independent reference apply: one thread per node gathers the rows of its
 // eight elements; optionally fused with the certification residual.
@@ -51,7 +53,8 @@ struct RefArgs {
 bool cubic(const pgj_grid& g);
 ElementCoef coef_of(const pgj_grid& g);
 int check_grid(const pgj_grid* g);
-int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgs& A, hipStream_t s);
+template <typename T>
+int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgsT<T>& A, hipStream_t s);
 int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_t s);
 int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, int soa, hipStream_t s);
 int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, int soa, hipStream_t s);

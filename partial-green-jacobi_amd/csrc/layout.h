@@ -196,18 +196,19 @@ inline FFTGeom fft_geom(const pgj_grid& g) {
 }
 
 // Plan buffer: twiddles for x (length M), y, z; then the 6-coefficient symbol.
+// rs is the real size: 8 for the FP64 plan, 4 for its FP32 copy.
 struct PlanLayout {
   size_t tw_x, tw_y, tw_z, sym, bytes;
 };
 
-inline PlanLayout plan_layout(const pgj_grid& g) {
+inline PlanLayout plan_layout(const pgj_grid& g, size_t rs = 8) {
   FFTGeom f = fft_geom(g);
   PlanLayout p{};
   size_t off = 0;
-  p.tw_x = off; off = align256(off + size_t(g.nx) * 16);
-  p.tw_y = off; off = align256(off + size_t(g.ny) * 16);
-  p.tw_z = off; off = align256(off + size_t(g.nz) * 16);
-  p.sym = off; off = align256(off + size_t(6) * f.nf * 8);
+  p.tw_x = off; off = align256(off + size_t(g.nx) * 2 * rs);
+  p.tw_y = off; off = align256(off + size_t(g.ny) * 2 * rs);
+  p.tw_z = off; off = align256(off + size_t(g.nz) * 2 * rs);
+  p.sym = off; off = align256(off + size_t(6) * f.nf * rs);
   p.bytes = off;
   return p;
 }
@@ -220,34 +221,37 @@ struct PrecondLayout {
   size_t bytes;
 };
 
-inline PrecondLayout precond_layout(const pgj_grid& g, int L) {
+inline PrecondLayout precond_layout(const pgj_grid& g, int L, size_t cs = 16) {
   FFTGeom f = fft_geom(g);
   PrecondLayout p{};
   size_t off = 0;
-  p.spec = off; off = align256(off + size_t(L) * 3 * f.nf * 16);
+  p.spec = off; off = align256(off + size_t(L) * 3 * f.nf * cs);
   p.pr = off; off = align256(off + size_t(L) * g.nz * 6 * 8);
   p.pg = off; off = align256(off + size_t(L) * f.nkb * g.ny * 4 * 8);
   p.bytes = off;
   return p;
 }
 
-// Resident workspace: preconditioner part + operator partials + certify partials.
+// Resident workspace: preconditioner part + operator partials + certify
+// partials + per-load refinement scales (cs: complex size of the spectrum).
 struct ResidentLayout {
   PrecondLayout pre;
   size_t ppq;       // L x hex8_blocks doubles
   size_t pcert;     // L x nblk x 4 doubles (sum u_c, |r|^2)
+  size_t rscale;    // L doubles (FP32 inner solver: |r| of each load's outer residual)
   int nblk_cert;
   size_t bytes;
 };
 
-inline ResidentLayout resident_layout(const pgj_grid& g, int L) {
+inline ResidentLayout resident_layout(const pgj_grid& g, int L, size_t cs = 16) {
   ResidentLayout r{};
-  r.pre = precond_layout(g, L);
+  r.pre = precond_layout(g, L, cs);
   size_t off = r.pre.bytes;
   r.ppq = off; off = align256(off + size_t(L) * hex8_blocks_bound(g) * 8);
   const int64_t n = int64_t(g.nx) * g.ny * g.nz;
   r.nblk_cert = int((n + 255) / 256);
   r.pcert = off; off = align256(off + size_t(L) * r.nblk_cert * 4 * 8);
+  r.rscale = off; off = align256(off + size_t(L) * 8);
   r.bytes = off;
   return r;
 }

@@ -34,6 +34,12 @@ void clear_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Complex value type of a real type: double2 for FP64, float2 for FP32.
+template <typename T> struct Complex;
+template <> struct Complex<double> { using type = double2; };
+template <> struct Complex<float> { using type = float2; };
+template <typename T> using cplx = typename Complex<T>::type;
+
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 // Design of load l when consecutive groups of `group` loads share one material

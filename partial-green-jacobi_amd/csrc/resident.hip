@@ -47,26 +47,32 @@ __global__ void certify_finish(const double* part, int nblk, StateView st) {
   if (threadIdx.x == 0) st.absolute[l] = sqrt(s);
 }
 
-struct ResidentBuffers {
+template <typename T>
+struct ResidentBuffersT {
   ResidentLayout lay;
   double* ppq;
   double* pcert;
-  double2* spec;
+  cplx<T>* spec;
   double* pr;
   double* pg;
+  double* rscale;
 };
+using ResidentBuffers = ResidentBuffersT<double>;
 
-// Workspace regions of a resident set of L loads; the per-load rows start at load l0.
-ResidentBuffers resident_buffers(const pgj_grid& g, int L, void* ws, int l0 = 0) {
-  ResidentBuffers b{};
-  b.lay = resident_layout(g, L);
+// Workspace regions of a resident set of L loads (spectrum in precision T);
+// the per-load rows start at load l0.
+template <typename T = double>
+ResidentBuffersT<T> resident_buffers(const pgj_grid& g, int L, void* ws, int l0 = 0) {
+  ResidentBuffersT<T> b{};
+  b.lay = resident_layout(g, L, sizeof(cplx<T>));
   const FFTGeom f = fft_geom(g);
   char* w = static_cast<char*>(ws);
   b.ppq = reinterpret_cast<double*>(w + b.lay.ppq) + int64_t(l0) * hex8_blocks_bound(g);
   b.pcert = reinterpret_cast<double*>(w + b.lay.pcert) + int64_t(l0) * b.lay.nblk_cert * 4;
-  b.spec = reinterpret_cast<double2*>(w + b.lay.pre.spec) + int64_t(l0) * 3 * f.nf;
+  b.spec = reinterpret_cast<cplx<T>*>(w + b.lay.pre.spec) + int64_t(l0) * 3 * f.nf;
   b.pr = reinterpret_cast<double*>(w + b.lay.pre.pr) + int64_t(l0) * g.nz * 6;
   b.pg = reinterpret_cast<double*>(w + b.lay.pre.pg) + int64_t(l0) * f.nkb * g.ny * 4;
+  b.rscale = reinterpret_cast<double*>(w + b.lay.rscale) + l0;
   return b;
 }
 
@@ -120,7 +126,7 @@ extern "C" int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, i
   for (int l = 0; l < nl; ++l)
     if (int e = launch_rhs(*g, a + (sl.design() + l / group) * N, macros + 6 * l, f + int64_t(l) * 3 * N, 1, s))
       return e;
-  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
+  ResidentBuffers b = resident_buffers<double>(*g, L, ws, l0);
   const int nblk = b.lay.nblk_cert;
   hipLaunchKernelGGL(square_partials, dim3(nblk, nl), dim3(256), 0, s, f, N, b.pcert);
   PGJ_LAUNCHED("square_partials");
@@ -137,7 +143,7 @@ extern "C" int pgj_resident_certify(const pgj_grid* g, int L, int group, int l0,
   if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
   PGJ_REQUIRE(a && u && rhs && r && state && ws, "invalid certify arguments");
   hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
+  ResidentBuffers b = resident_buffers<double>(*g, L, ws, l0);
   double* us = u + sl.vec();
   if (int e = launch_component_sums(*g, nl, us, b.pcert, 1, s)) return e;
   if (int e = launch_subtract_mean(*g, nl, us, b.pcert, 1, s)) return e;
@@ -151,14 +157,86 @@ extern "C" int pgj_resident_certify(const pgj_grid* g, int L, int group, int l0,
 }
 
 namespace pgj {
-int fxy(FFTArgs& A, const pgj_grid& g, int L, int mode, hipStream_t s) {
-  return static_fft(g) ? launch_fxy_static(A, L, mode, s) : launch_fxy_mode(A, L, mode, s);
+// FFT/Green stage launchers: compile-time shapes where they exist; FP64 also
+// has the runtime-shaped kernels, FP32 does not.
+template <typename T>
+int fxy(FFTArgsT<T>& A, const pgj_grid& g, int L, int mode, hipStream_t s) {
+  if (static_fft(g)) return launch_fxy_static(A, L, mode, s);
+  if constexpr (sizeof(T) == 8) return launch_fxy_mode(A, L, mode, s);
+  set_error("FP32 FFT kernels exist for cubic n in 16, 24, 32, 48, 64, 96, 128 only");
+  return PGJ_ERR_UNSUPPORTED;
 }
-int fz(FFTArgs& A, const pgj_grid& g, int L, int mode, hipStream_t s) {
-  return static_fft(g) ? launch_fz_static(A, L, mode, s) : launch_fz_mode(A, L, mode, s);
+template <typename T>
+int fz(FFTArgsT<T>& A, const pgj_grid& g, int L, int mode, hipStream_t s) {
+  if (static_fft(g)) return launch_fz_static(A, L, mode, s);
+  if constexpr (sizeof(T) == 8) return launch_fz_mode(A, L, mode, s);
+  set_error("FP32 FFT kernels exist for cubic n in 16, 24, 32, 48, 64, 96, 128 only");
+  return PGJ_ERR_UNSUPPORTED;
 }
-int fxyi(FFTArgs& A, const pgj_grid& g, int L, hipStream_t s) {
-  return static_fft(g) ? launch_fxyi_static(A, L, s) : launch_fxyi_mode(A, L, kFxyiPCG, s);
+template <typename T>
+int fxyi(FFTArgsT<T>& A, const pgj_grid& g, int L, hipStream_t s) {
+  if (static_fft(g)) return launch_fxyi_static(A, L, s);
+  if constexpr (sizeof(T) == 8) return launch_fxyi_mode(A, L, kFxyiPCG, s);
+  set_error("FP32 FFT kernels exist for cubic n in 16, 24, 32, 48, 64, 96, 128 only");
+  return PGJ_ERR_UNSUPPORTED;
+}
+
+// z = M^-1 r on a slice with the active-mask set-up (precision T); A carries the plan.
+template <typename T>
+int resident_init_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgsT<T> A, const T* sc,
+                    const cplx<T>* s_hat, const T* r, T* z, void* state, void* ws, hipStream_t s) {
+  Slice sl{};
+  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(r && z && state && ws, "invalid init arguments");
+  PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
+  ResidentBuffersT<T> b = resident_buffers<T>(*g, L, ws, l0);
+  A.rin = r + sl.vec();
+  A.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
+  A.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
+  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z + sl.vec();
+  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, nl); A.ppq_stride = hex8_blocks_bound(*g);
+  A.st = state_view(state, L, l0);
+  A.soa = 1;
+  A.group = group;
+  if (int e = fxy(A, *g, nl, kFxyPre, s)) return e;
+  if (int e = fz(A, *g, nl, kFzInit, s)) return e;
+  return fxyi(A, *g, nl, s);
+}
+
+// One of the four fused iteration stages on a slice (precision T).
+template <typename T>
+int resident_stage_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgsT<T> F, const T* a, const T* sc,
+                     const cplx<T>* s_hat, T* u, T* r, T* pold, T* pnew, T* q, T* z, void* state, void* ws,
+                     int stage, hipStream_t s) {
+  Slice sl{};
+  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(a && u && r && pold && pnew && q && z && state && ws, "invalid stage arguments");
+  PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
+  PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
+  PGJ_REQUIRE(rowmarch_config(*g, nl).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
+  ResidentBuffersT<T> b = resident_buffers<T>(*g, L, ws, l0);
+  StateView st = state_view(state, L, l0);
+  const int64_t v = sl.vec();
+  F.u = u + v; F.r = r + v; F.q = q + v; F.p = pnew + v;
+  F.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
+  F.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
+  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z + v;
+  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl); F.ppq_stride = hex8_blocks_bound(*g);
+  F.st = st;
+  F.soa = 1;
+  F.group = group;
+  if (stage == 0) {
+    const int pg_count = F.f.nkb * g->ny;
+    RowArgsT<T> K{};
+    K.a = a + sl.design() * sl.N; K.u = z + v; K.out = q + v; K.pold = pold + v; K.pnew = pnew + v; K.st = st;
+    K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
+    K.ppq = b.ppq; K.ppq_stride = F.ppq_stride;
+    K.group = group;
+    return launch_rowmarch(*g, nl, kModePCG, K, s);
+  }
+  if (stage == 1) return fxy(F, *g, nl, kFxyPCG, s);
+  if (stage == 2) return fz(F, *g, nl, kFzPCG, s);
+  return fxyi(F, *g, nl, s);
 }
 }  // namespace pgj
 
@@ -168,7 +246,7 @@ extern "C" int pgj_resident_scale(const pgj_grid* g, const void* plan, const dou
   PGJ_REQUIRE(plan && a && sc && s_hat && ws, "invalid resident scale arguments");
   PGJ_REQUIRE(alpha >= 0.0 && alpha <= 1.0, "alpha must be in [0, 1], got %g", alpha);
   hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, 1, ws);
+  ResidentBuffers b = resident_buffers<double>(*g, 1, ws);
   if (int e = launch_scale(*g, a, alpha, nullptr, sc, b.pcert, 1, s)) return e;
   FFTArgs A = fft_args(*g, plan);
   A.s = sc; A.spec = static_cast<double2*>(s_hat); A.pr = b.pr; A.pg = b.pg; A.soa = 1;
@@ -180,24 +258,9 @@ extern "C" int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, in
                                  const double* sc, const void* s_hat, const double* r, double* z, void* state,
                                  void* ws, void* stream) {
   if (int e = check_fft(g)) return e;
-  Slice sl{};
-  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
-  PGJ_REQUIRE(plan && r && z && state && ws, "invalid init arguments");
-  PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
-  hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
-  FFTArgs A = fft_args(*g, plan);
-  A.rin = r + sl.vec();
-  A.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
-  A.shat = s_hat ? static_cast<const double2*>(s_hat) + sl.design() * 3 * sl.nf : nullptr;
-  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z + sl.vec();
-  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, nl); A.ppq_stride = hex8_blocks_bound(*g);
-  A.st = state_view(state, L, l0);
-  A.soa = 1;
-  A.group = group;
-  if (int e = fxy(A, *g, nl, kFxyPre, s)) return e;
-  if (int e = fz(A, *g, nl, kFzInit, s)) return e;
-  return fxyi(A, *g, nl, s);
+  PGJ_REQUIRE(plan, "invalid init arguments");
+  return resident_init_t<double>(g, L, group, l0, nl, fft_args(*g, plan), sc, static_cast<const double2*>(s_hat), r,
+                                 z, state, ws, as_stream(stream));
 }
 
 extern "C" int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
@@ -205,37 +268,9 @@ extern "C" int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, i
                                   double* pold, double* pnew, double* q, double* z, void* state, void* ws, int stage,
                                   void* stream) {
   if (int e = check_fft(g)) return e;
-  Slice sl{};
-  if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
-  PGJ_REQUIRE(plan && a && u && r && pold && pnew && q && z && state && ws, "invalid stage arguments");
-  PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
-  PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
-  PGJ_REQUIRE(rowmarch_config(*g, nl).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
-  hipStream_t s = as_stream(stream);
-  ResidentBuffers b = resident_buffers(*g, L, ws, l0);
-  StateView st = state_view(state, L, l0);
-  const int64_t v = sl.vec();
-  FFTArgs F = fft_args(*g, plan);
-  F.u = u + v; F.r = r + v; F.q = q + v; F.p = pnew + v;
-  F.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
-  F.shat = s_hat ? static_cast<const double2*>(s_hat) + sl.design() * 3 * sl.nf : nullptr;
-  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z + v;
-  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl); F.ppq_stride = hex8_blocks_bound(*g);
-  F.st = st;
-  F.soa = 1;
-  F.group = group;
-  if (stage == 0) {
-    const int pg_count = F.f.nkb * g->ny;
-    RowArgs K{};
-    K.a = a + sl.design() * sl.N; K.u = z + v; K.out = q + v; K.pold = pold + v; K.pnew = pnew + v; K.st = st;
-    K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
-    K.ppq = b.ppq; K.ppq_stride = F.ppq_stride;
-    K.group = group;
-    return launch_rowmarch(*g, nl, kModePCG, K, s);
-  }
-  if (stage == 1) return fxy(F, *g, nl, kFxyPCG, s);
-  if (stage == 2) return fz(F, *g, nl, kFzPCG, s);
-  return fxyi(F, *g, nl, s);
+  PGJ_REQUIRE(plan, "invalid stage arguments");
+  return resident_stage_t<double>(g, L, group, l0, nl, fft_args(*g, plan), a, sc, static_cast<const double2*>(s_hat),
+                                  u, r, pold, pnew, q, z, state, ws, stage, as_stream(stream));
 }
 
 extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
@@ -249,6 +284,44 @@ extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0,
     for (int stage = 0; stage < 4; ++stage)
       if (int e = pgj_resident_stage(g, L, group, l0, nl, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws,
                                      stage, stream))
+        return e;
+  }
+  return PGJ_OK;
+}
+
+// ------------------------------------------------- FP32 inner resident solver
+namespace pgj {
+FFTArgsT<float> fft_args32(const pgj_grid& g, const void* plan32);
+}
+
+extern "C" int pgj_inner_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32,
+                              const float* sc, const void* s_hat, const float* r, float* z, void* state, void* ws,
+                              void* stream) {
+  PGJ_REQUIRE(g && plan32, "invalid inner init arguments");
+  PGJ_REQUIRE(static_cube(*g), "the FP32 inner solver needs a cubic n in 16, 24, 32, 48, 64, 96, 128");
+  return resident_init_t<float>(g, L, group, l0, nl, fft_args32(*g, plan32), sc, static_cast<const float2*>(s_hat), r,
+                                z, state, ws, as_stream(stream));
+}
+
+extern "C" int pgj_inner_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32,
+                               const float* a, const float* sc, const void* s_hat, float* u, float* r, float* pold,
+                               float* pnew, float* q, float* z, void* state, void* ws, int stage, void* stream) {
+  PGJ_REQUIRE(g && plan32, "invalid inner stage arguments");
+  PGJ_REQUIRE(static_cube(*g), "the FP32 inner solver needs a cubic n in 16, 24, 32, 48, 64, 96, 128");
+  return resident_stage_t<float>(g, L, group, l0, nl, fft_args32(*g, plan32), a, sc, static_cast<const float2*>(s_hat),
+                                 u, r, pold, pnew, q, z, state, ws, stage, as_stream(stream));
+}
+
+extern "C" int pgj_inner_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32,
+                                 const float* a, const float* sc, const void* s_hat, float* u, float* r, float* p0,
+                                 float* p1, float* q, float* z, void* state, void* ws, int iters, void* stream) {
+  PGJ_REQUIRE(iters >= 2 && iters % 2 == 0, "iteration count must be even and >= 2, got %d", iters);
+  for (int i = 0; i < iters; ++i) {
+    float* pold = (i % 2 == 0) ? p0 : p1;
+    float* pnew = (i % 2 == 0) ? p1 : p0;
+    for (int stage = 0; stage < 4; ++stage)
+      if (int e = pgj_inner_stage(g, L, group, l0, nl, plan32, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws,
+                                  stage, stream))
         return e;
   }
   return PGJ_OK;

@@ -7,48 +7,51 @@
 
 namespace pgj {
 
+// Complex helpers for double2 (FP64) and float2 (FP32) values (cplx<T> in pgj_internal.h).
 __device__ __forceinline__ double2 c2(double x, double y) { return make_double2(x, y); }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return c2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return c2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+__device__ __forceinline__ float2 c2(float x, float y) { return make_float2(x, y); }
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return c2(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return c2(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
   return c2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
-__device__ __forceinline__ double2 conjd(double2 a) { return c2(a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C conjd(C a) { return c2(a.x, -a.y); }
 
 // W_n^k from a table of length T = n * step (W_T[j] = exp(-2 pi i j / T)).
-template <bool INV>
-__device__ __forceinline__ double2 twiddle(const double2* tw, int k, int step) {
-  const double2 t = tw[k * step];
+template <bool INV, typename C>
+__device__ __forceinline__ C twiddle(const C* tw, int k, int step) {
+  const C t = tw[k * step];
   return INV ? conjd(t) : t;
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void dft(double2 (&v)[R], const double2* tw, int n, int step) {
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void dft(C (&v)[R], const C* tw, int n, int step) {
+  using T = decltype(C{}.x);
   if constexpr (R == 2) {
-    const double2 a = v[0];
+    const C a = v[0];
     v[0] = cadd(a, v[1]);
     v[1] = csub(a, v[1]);
   } else if constexpr (R == 4) {
-    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]);
-    double2 t3 = csub(v[1], v[3]);
+    const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]);
+    C t3 = csub(v[1], v[3]);
     t3 = INV ? c2(-t3.y, t3.x) : c2(t3.y, -t3.x);
     v[0] = cadd(t0, t2);
     v[2] = csub(t0, t2);
     v[1] = cadd(t1, t3);
     v[3] = csub(t1, t3);
   } else if constexpr (R == 3) {
-    constexpr double h = 0.86602540378443864676;  // sqrt(3)/2
-    const double2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-    const double2 m = c2(v[0].x - 0.5 * s.x, v[0].y - 0.5 * s.y);
-    const double2 jd = INV ? c2(-h * d.y, h * d.x) : c2(h * d.y, -h * d.x);
+    constexpr T h = T(0.86602540378443864676);  // sqrt(3)/2
+    const C s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    const C m = c2(v[0].x - T(0.5) * s.x, v[0].y - T(0.5) * s.y);
+    const C jd = INV ? c2(-h * d.y, h * d.x) : c2(h * d.y, -h * d.x);
     v[0] = cadd(v[0], s);
     v[1] = cadd(m, jd);
     v[2] = csub(m, jd);
   } else {
-    double2 out[R];
+    C out[R];
 #pragma unroll
     for (int p = 0; p < R; ++p) {
-      double2 acc = v[0];
+      C acc = v[0];
 #pragma unroll
       for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(v[q], twiddle<INV>(tw, ((p * q) % R) * (n / R), step)));
       out[p] = acc;
@@ -94,10 +97,10 @@ struct Lines {
   __device__ static __forceinline__ int base(int l) { return (l / PER) * OUTER + (l % PER) * INNER; }
 };
 
-template <int R, bool INV, int N, int NS, class LN, int THREADS>
-__device__ __forceinline__ void sstage(double2* buf, const double2* __restrict__ tw, int step) {
+template <int R, bool INV, int N, int NS, class LN, int THREADS, typename C>
+__device__ __forceinline__ void sstage(C* buf, const C* __restrict__ tw, int step) {
   constexpr int NB = N / R, COUNT = LN::nlines * NB, PT = (COUNT + THREADS - 1) / THREADS, M = N / (NS * R);
-  double2 v[PT][R];
+  C v[PT][R];
 #pragma unroll
   for (int i = 0; i < PT; ++i) {
     const int b = threadIdx.x + i * THREADS;
@@ -105,10 +108,10 @@ __device__ __forceinline__ void sstage(double2* buf, const double2* __restrict__
       const int line = LN::line_fast ? b % LN::nlines : b / NB;
       const int j = LN::line_fast ? b / LN::nlines : b % NB;
       const int k = j % NS;
-      const double2* L = buf + LN::base(line);
+      const C* L = buf + LN::base(line);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double2 x = L[(j + r * NB) * LN::es];
+        const C x = L[(j + r * NB) * LN::es];
         v[i][r] = (NS == 1 || r == 0) ? x : cmul(x, twiddle<INV>(tw, k * r * M, step));
       }
     }
@@ -122,7 +125,7 @@ __device__ __forceinline__ void sstage(double2* buf, const double2* __restrict__
       const int j = LN::line_fast ? b / LN::nlines : b % NB;
       const int k = j % NS;
       dft<R, INV>(v[i], tw, N, step);
-      double2* L = buf + LN::base(line);
+      C* L = buf + LN::base(line);
       const int idx = (j / NS) * NS * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) L[(idx + r * NS) * LN::es] = v[i][r];
@@ -131,8 +134,8 @@ __device__ __forceinline__ void sstage(double2* buf, const double2* __restrict__
   __syncthreads();
 }
 
-template <bool INV, int N, class LN, int THREADS, int S = 0>
-__device__ __forceinline__ void sfft(double2* buf, const double2* __restrict__ tw, int step) {
+template <bool INV, int N, class LN, int THREADS, int S = 0, typename C>
+__device__ __forceinline__ void sfft(C* buf, const C* __restrict__ tw, int step) {
   if constexpr (S < radix_count(N)) {
     sstage<radix_at(N, S), INV, N, radix_ns(N, S), LN, THREADS>(buf, tw, step);
     sfft<INV, N, LN, THREADS, S + 1>(buf, tw, step);

@@ -86,6 +86,21 @@ SIGNATURES = {
     'pgj_gauss_ratio': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_gauss_candidate': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     'pgj_gauss_bisect': (c_int, [c_int, c_void_p, c_void_p, c_double, c_void_p]),
+    'pgj_plan32_bytes': (c_size_t, [GP]),
+    'pgj_plan32_init': (c_int, [GP, c_void_p, c_void_p, c_void_p]),
+    'pgj_to_f32': (c_int, [c_int64, c_void_p, c_void_p, c_void_p]),
+    'pgj_inner_ws_bytes': (c_size_t, [GP, c_int]),
+    'pgj_inner_begin': (c_int, [GP, c_int, c_int, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p]),
+    'pgj_inner_init': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p]),
+    'pgj_inner_stage': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                c_void_p]),
+    'pgj_inner_iterate': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                  c_void_p]),
+    'pgj_inner_end': (c_int, [GP, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_pcg_ws_bytes': (c_size_t, [c_int, c_int64]),
     'pgj_pcg_start': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_pcg_init': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

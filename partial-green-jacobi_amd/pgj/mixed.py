@@ -1,0 +1,196 @@
+"""Mixed-precision form: FP32 inner PCG inside FP64 iterative refinement.
+
+Each outer step certifies the FP64 iterate (u <- P u, r = f - K u with the
+FP64 reference operator, |r| per load), stops the loads whose |r| meets
+max(rtol |f|, 1e-13/sqrt(n)), and solves K d = r / |r| for the others with the
+resident PCG in FP32 (FP32 row-march HEX8 apply, FP32 Stockham FFT and Green
+multiply, FP64 reductions and scalars) to relative tolerance ``inner_rtol``;
+then u += |r| d in FP64. The residual, the correction accumulation, the
+certification dot products, the geometric-mean diagonal normalisation of the
+scale and C^H stay FP64. Every returned solution passes the same FP64
+true-residual certification as the plain path (``resident.py:186-209``
+semantics; up to ``max_outer`` refinement steps instead of two restarts).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
+
+LOADS6 = np.eye(6)
+
+
+class MixedResident:
+    """D designs x G loads of one cubic n^3 grid solved by FP32-inner / FP64-outer refinement."""
+
+    def __init__(self, n, loads=LOADS6, designs=1, inner_rtol=1e-4, chunk=4, max_outer=30, lam=LAMBDA_STUDY2,
+                 mu=MU_STUDY2, device='cuda'):
+        if chunk < 2 or chunk % 2:
+            raise ValueError('graph chunk must be even and >= 2')
+        if not 0 < inner_rtol < 1:
+            raise ValueError('inner_rtol must be in (0, 1)')
+        self.loads = np.atleast_2d(np.asarray(loads, dtype=np.float64))
+        self.n, self.D, self.G = int(n), int(designs), len(self.loads)
+        self.L = self.D * self.G
+        self.inner_rtol, self.chunk, self.max_outer = float(inner_rtol), chunk, int(max_outer)
+        shape = (self.n,) * 3
+        self.op = Operator(torch.ones(shape, dtype=torch.float64, device=device), None, lam, mu, device).enable_green()
+        g = self.op.grid
+        lib = _lib.lib()
+        N = self.n ** 3
+        nf = self.n * self.n * (self.n // 2 + 1)
+        self.N, L = N, self.L
+        dev = torch.device(device)
+        f64 = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        f32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
+        # FP64: fields, scale, outer vectors (SoA), certification state
+        self.a = torch.ones((self.D, N), dtype=torch.float64, device=dev)
+        self.scale = torch.ones((self.D, 3 * N), dtype=torch.float64, device=dev)
+        self.scale_hat = torch.zeros((self.D, 3 * nf), dtype=torch.complex128, device=dev)
+        self.u, self.r, self.rhs = f64(L, 3 * N), f64(L, 3 * N), f64(L, 3 * N)
+        self.state = torch.zeros(lib.pgj_resident_state_bytes(L), dtype=torch.uint8, device=dev)
+        self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, L), dtype=torch.uint8, device=dev)
+        self.scale_ws = torch.empty(lib.pgj_resident_ws_bytes(g, 1), dtype=torch.uint8, device=dev)
+        # FP32: plan, fields, inner vectors, inner state
+        self.plan32 = torch.empty(lib.pgj_plan32_bytes(g), dtype=torch.uint8, device=dev)
+        _lib.call('pgj_plan32_init', g, _lib.ptr(self.op._plan), _lib.ptr(self.plan32), _lib.stream())
+        self.a32 = torch.ones((self.D, N), dtype=torch.float32, device=dev)
+        self.scale32 = torch.ones((self.D, 3 * N), dtype=torch.float32, device=dev)
+        self.scale_hat32 = torch.zeros((self.D, 3 * nf), dtype=torch.complex64, device=dev)
+        self.d, self.r32, self.p0, self.p1, self.q, self.z = (f32(L, 3 * N) for _ in range(6))
+        self.state32 = torch.zeros(lib.pgj_resident_state_bytes(L), dtype=torch.uint8, device=dev)
+        self.ws32 = torch.empty(lib.pgj_inner_ws_bytes(g, L), dtype=torch.uint8, device=dev)
+        offs = (_lib.c_size_t * 7)()
+        _lib.check(lib.pgj_resident_state_offsets(L, offs))
+        o = list(offs)
+        view = lambda st, k, dt, w: st[o[k]:o[k] + w * L].view(dt)
+        self.target2 = view(self.state, 1, torch.float64, 8)
+        self.norm = view(self.state, 2, torch.float64, 8)
+        self.absolute = view(self.state, 3, torch.float64, 8)
+        self.active32 = view(self.state32, 5, torch.uint8, 1)
+        self.iterations32 = view(self.state32, 4, torch.int64, 8)
+        self.bad32 = view(self.state32, 6, torch.uint8, 1)
+        self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
+        self.aos = f64(self.G, 3 * N)
+        self.graphs = None
+
+    # -- inputs -----------------------------------------------------------------------
+    def material(self, d, a):
+        self.a[d].copy_(torch.as_tensor(a, dtype=torch.float64).reshape(-1))
+        _lib.call('pgj_to_f32', self.N, _lib.ptr(self.a[d]), _lib.ptr(self.a32[d]), _lib.stream())
+
+    def select(self, d, alpha):
+        """FP64 scale (geometric-mean normalised diagonal) and spectrum, then their FP32 copies."""
+        g = self.op.grid
+        _lib.call('pgj_resident_scale', g, _lib.ptr(self.op._plan), _lib.ptr(self.a[d]), float(alpha),
+                  _lib.ptr(self.scale[d]), _lib.ptr(self.scale_hat[d]), _lib.ptr(self.scale_ws), _lib.stream())
+        _lib.call('pgj_to_f32', 3 * self.N, _lib.ptr(self.scale[d]), _lib.ptr(self.scale32[d]), _lib.stream())
+        _lib.call('pgj_to_f32', 2 * self.scale_hat.shape[1], _lib.ptr(self.scale_hat[d]),
+                  _lib.ptr(self.scale_hat32[d]), _lib.stream())
+
+    # -- launches ---------------------------------------------------------------------
+    def setup(self, rtol):
+        m = np.tile(self.loads, (self.D, 1)).reshape(-1)
+        _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.a), _lib.host_doubles(m),
+                  float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
+
+    def certify(self):
+        _lib.call('pgj_resident_certify', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.a), _lib.ptr(self.u),
+                  _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
+
+    def begin(self):
+        _lib.call('pgj_inner_begin', self.op.grid, self.L, 0, self.L, _lib.ptr(self.r), _lib.ptr(self.state),
+                  self.inner_rtol, _lib.ptr(self.r32), _lib.ptr(self.d), _lib.ptr(self.state32), _lib.ptr(self.ws32),
+                  _lib.stream())
+        _lib.call('pgj_inner_init', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.plan32),
+                  _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.r32), _lib.ptr(self.z),
+                  _lib.ptr(self.state32), _lib.ptr(self.ws32), _lib.stream())
+
+    def iteration(self):
+        _lib.call('pgj_inner_iterate', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.plan32),
+                  _lib.ptr(self.a32), _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.d),
+                  _lib.ptr(self.r32), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.z),
+                  _lib.ptr(self.state32), _lib.ptr(self.ws32), self.chunk, _lib.stream())
+
+    def end_and_certify(self):
+        _lib.call('pgj_inner_end', self.op.grid, self.L, 0, self.L, _lib.ptr(self.d), _lib.ptr(self.ws32),
+                  _lib.ptr(self.u), _lib.stream())
+        self.certify()
+
+    def capture(self):
+        self.setup(1e-8)
+        self.u.normal_()
+        self.certify()
+        self.begin()
+        self.iteration()
+        self.end_and_certify()
+        torch.cuda.synchronize()
+        self.graphs = {}
+        for name in ('begin', 'iteration', 'end_and_certify'):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                getattr(self, name)()
+            self.graphs[name] = graph
+        torch.cuda.synchronize()
+        self.u.zero_()
+        return self
+
+    def _run(self, name):
+        if self.graphs is None:
+            getattr(self, name)()
+        else:
+            self.graphs[name].replay()
+
+    # -- the solve --------------------------------------------------------------------
+    def solve(self, rtol=1e-8):
+        """Refine every load to max(rtol |f|, 1e-13/sqrt(n)) from the current u; returns (infos, stats)."""
+        self.setup(rtol)
+        self.certify()
+        inner_total = np.zeros(self.L, dtype=np.int64)
+        outer = host_checks = 0
+        while True:
+            v = torch.stack((self.norm, self.absolute, self.target2.sqrt()), dim=1).cpu().numpy()
+            ok = np.isfinite(v[:, 1]) & (v[:, 1] <= v[:, 2])
+            if ok.all():
+                break
+            if outer >= self.max_outer or not np.all(np.isfinite(v[:, 1])):
+                raise RuntimeError(('mixed-precision refinement did not certify', outer, v.tolist()))
+            outer += 1
+            self._run('begin')
+            while True:
+                host_checks += 1
+                if not bool(self.active32.any().item()):
+                    break
+                self._run('iteration')
+            if bool(self.bad32.any().item()):
+                raise RuntimeError('FP32 inner PCG broke down (non-positive or non-finite curvature)')
+            inner_total += self.iterations32.cpu().numpy()
+            self._run('end_and_certify')
+        infos = [dict(rhs_norm=float(row[0]), absolute_residual=float(row[1]), threshold=float(row[2]),
+                      iterations=int(inner_total[j]), relative_residual=float(row[1] / max(row[0], 1e-300)),
+                      converged=True, outer=outer) for j, row in enumerate(v)]
+        return infos, dict(outer=outer, inner=int(inner_total.sum()), host_checks=host_checks)
+
+    # -- results ------------------------------------------------------------------------
+    def solutions(self, d):
+        _lib.call('pgj_soa_to_aos', self.op.grid, self.G, _lib.ptr(self.u[d * self.G:(d + 1) * self.G]),
+                  _lib.ptr(self.aos), _lib.stream())
+        return self.aos.clone()
+
+    def effective_tensor(self, d=0):
+        """C^H (6, 6) of design d from the FP64 solutions (``reproduction.py:166-176``)."""
+        if self.G != 6 or not np.array_equal(self.loads, LOADS6):
+            raise ValueError('C^H needs the six unit Voigt strains')
+        sol = self.solutions(d)
+        out = torch.empty(36, dtype=torch.float64, device=self.a.device)
+        _lib.call('pgj_effective_tensor', self.op.grid, _lib.ptr(self.a[d]), _lib.ptr(sol), _lib.ptr(self.ch_ws),
+                  _lib.ptr(out), _lib.stream())
+        return out.reshape(6, 6)
+
+    def energies(self, d=0):
+        """FP64 element energies (G, N) of design d's solutions."""
+        sol = self.solutions(d)
+        out = torch.empty((self.G, self.N), dtype=torch.float64, device=sol.device)
+        _lib.call('pgj_hex8_energy', self.op.grid, self.G, _lib.ptr(sol), _lib.host_doubles(self.loads.reshape(-1)),
+                  _lib.ptr(out), _lib.stream())
+        return out
