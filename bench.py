@@ -48,6 +48,9 @@ def parse():
                         'captured graph vs the same designs solved one at a time; serve: Poisson-arrival '
                         'design-step requests through the continuous-batching server; mixed: FP32-inner / '
                         'FP64-refinement solves at 64^3 and 128^3 against the plain FP64 solve')
+    p.add_argument('--profile-stages', action='store_true',
+                   help='only launch a 256 MiB device copy (counter calibration) and the four iteration stages '
+                        '--probe-iters times each at the bench workload, for rocprofv3 counter runs')
     p.add_argument('--serve-seconds', type=float, default=20.0, help='arrival window of the serve form')
     p.add_argument('--serve-rate', type=float, default=20.0)
     p.add_argument('--serve-clients', type=int, default=64)
@@ -128,6 +131,23 @@ def cpu_baseline(a, alpha, its_total, seconds):
                       f'step at {a.shape[0]}^3 (numpy FP64, pocketfft single-threaded, BLAS {threads} threads), '
                       f'{t_iter * 1e3:.1f} ms/iteration, extrapolated to the GPU step\'s {its_total} total '
                       f'iterations plus 12 certification applies'}
+
+
+def profile_stages(args):
+    """Counter-run workload: one 256 MiB copy (read 256 MiB, write 256 MiB) and the four stages."""
+    import pgj
+    from pgj.density import filtered_noise, simp
+
+    src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    n = args.n
+    res = pgj.Resident(n, np.eye(6), 'parallel', batch=args.batch).capture()
+    res.material(torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda'))
+    res.select(args.alpha)
+    durations, active = probe_stages(res, args.probe_iters)
+    print(json.dumps({'stages_ms': durations, 'active': active, 'copy_bytes': 2 * src.numel() * 8}), flush=True)
 
 
 def batched_form(args):
@@ -307,6 +327,9 @@ def main():
     if args.form == 'mixed':
         torch.cuda.set_device(0)
         return mixed_form(args)
+    if args.profile_stages:
+        torch.cuda.set_device(0)
+        return profile_stages(args)
     import pgj
     from pgj.density import filtered_noise
     from pgj.outer import PaperOuter
