@@ -92,13 +92,15 @@ int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alpha, double*
                        void* stream);
 
 /* Row-march launch shape of the HEX8 kernel for grid g with L loads per
- * launch: R element rows per wave (1 or 2), W row waves per block (1..8), ZC
- * z-planes marched per block. set_shape records a tuned shape (all three 0
+ * launch in precision es (8: FP64, 4: FP32): R element rows per wave (1 or
+ * 2), W row waves per block (1..8), ZC z-planes marched per block, plane = 1
+ * for the plane-carry march (in-plane butterflies shared by the two element
+ * layers of a node plane). set_shape records a tuned shape (R = W = ZC = 0
  * clears it); get_shape reports the shape in use and the blocks per load
- * (out4 = R, W, ZC, blocks). Counterpart of the reference's tile/variant
+ * (out5 = R, W, ZC, plane, blocks). Counterpart of the reference's tile/variant
  * sweep (solver/src/hex8_modal/modal_adapter.py:28-41, gpu_study.py:138-142). */
-int pgj_hex8_set_shape(const pgj_grid* g, int L, int R, int W, int ZC);
-int pgj_hex8_get_shape(const pgj_grid* g, int L, int* out4);
+int pgj_hex8_set_shape(const pgj_grid* g, int L, int es, int R, int W, int ZC, int plane);
+int pgj_hex8_get_shape(const pgj_grid* g, int L, int es, int* out5);
 
 /* Layout conversions between AoS (L, N, 3) and SoA (L, 3, N) vectors. */
 int pgj_aos_to_soa(const pgj_grid* g, int L, const double* in, double* out, void* stream);

@@ -26,40 +26,41 @@ void set_error(const char* fmt, ...) {
 void clear_error() { g_error.clear(); }
 
 namespace {
-using ShapeKey = std::tuple<int, int, int, int>;  // nz, ny, nx, L
+using ShapeKey = std::tuple<int, int, int, int, int>;  // nz, ny, nx, L, element bytes
 std::mutex g_shape_mutex;
-std::map<ShapeKey, std::tuple<int, int, int>> g_shapes;
+std::map<ShapeKey, std::tuple<int, int, int, int>> g_shapes;
 }  // namespace
 
-bool tuned_shape(const pgj_grid& g, int L, int* R, int* W, int* ZC) {
+bool tuned_shape(const pgj_grid& g, int L, int es, int* R, int* W, int* ZC, int* plane) {
   std::lock_guard<std::mutex> lock(g_shape_mutex);
-  auto it = g_shapes.find(ShapeKey(g.nz, g.ny, g.nx, L));
+  auto it = g_shapes.find(ShapeKey(g.nz, g.ny, g.nx, L, es));
   if (it == g_shapes.end()) return false;
-  std::tie(*R, *W, *ZC) = it->second;
+  std::tie(*R, *W, *ZC, *plane) = it->second;
   return true;
 }
 
 }  // namespace pgj
 
-extern "C" int pgj_hex8_set_shape(const pgj_grid* g, int L, int R, int W, int ZC) {
-  PGJ_REQUIRE(g && L >= 1, "invalid shape key");
+extern "C" int pgj_hex8_set_shape(const pgj_grid* g, int L, int es, int R, int W, int ZC, int plane) {
+  PGJ_REQUIRE(g && L >= 1 && (es == 4 || es == 8), "invalid shape key");
   std::lock_guard<std::mutex> lock(pgj::g_shape_mutex);
-  const pgj::ShapeKey key(g->nz, g->ny, g->nx, L);
+  const pgj::ShapeKey key(g->nz, g->ny, g->nx, L, es);
   if (R == 0 && W == 0 && ZC == 0) {
     pgj::g_shapes.erase(key);
     return PGJ_OK;
   }
-  PGJ_REQUIRE((R == 1 || R == 2) && W >= 1 && W <= 8 && ZC >= 1 && ZC <= g->nz,
-              "row-march shape needs R in {1,2}, W in 1..8, ZC in 1..nz, got (%d,%d,%d)", R, W, ZC);
-  pgj::g_shapes[key] = std::make_tuple(R, W, ZC);
+  PGJ_REQUIRE((R == 1 || R == 2) && W >= 1 && W <= 8 && ZC >= 1 && ZC <= g->nz && (plane == 0 || plane == 1),
+              "row-march shape needs R in {1,2}, W in 1..8, ZC in 1..nz, plane in {0,1}, got (%d,%d,%d,%d)", R, W,
+              ZC, plane);
+  pgj::g_shapes[key] = std::make_tuple(R, W, ZC, plane);
   return PGJ_OK;
 }
 
-extern "C" int pgj_hex8_get_shape(const pgj_grid* g, int L, int* out4) {
-  PGJ_REQUIRE(g && L >= 1 && out4, "invalid shape query");
-  const pgj::RowMarch c = pgj::rowmarch_config(*g, L);
+extern "C" int pgj_hex8_get_shape(const pgj_grid* g, int L, int es, int* out5) {
+  PGJ_REQUIRE(g && L >= 1 && (es == 4 || es == 8) && out5, "invalid shape query");
+  const pgj::RowMarch c = pgj::rowmarch_config(*g, L, es);
   PGJ_REQUIRE(c.valid, "grid has no row-march shape");
-  out4[0] = c.R; out4[1] = c.W; out4[2] = c.ZC; out4[3] = c.by * c.bz;
+  out5[0] = c.R; out5[1] = c.W; out5[2] = c.ZC; out5[3] = c.plane; out5[4] = c.by * c.bz;
   return PGJ_OK;
 }
 

@@ -20,7 +20,13 @@ struct Rows {
 // halo wave computes the element row below the block, and contributions to a
 // neighbour wave's first node row go through a double-buffered LDS slot. The
 // next plane's inputs are fetched before the current layer is computed.
-template <int MODE, int XPL, int R, bool CUBIC, int MAXT, typename T>
+// PLANE selects the plane-carry march: the in-plane butterflies of each node
+// plane are computed once and shared by the two element layers that touch it
+// (forward: the top plane of layer z is the bottom plane of layer z+1;
+// transposed: the modal forces of both layers on a plane are summed before
+// one in-plane transform), which halves the in-plane butterflies, the x
+// neighbour exchanges and the LDS traffic per element.
+template <int MODE, int XPL, int R, bool CUBIC, int MAXT, typename T, bool PLANE>
 __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   // the drop-in apply works on AoS vectors, the resident iteration on SoA
   constexpr bool SOA = MODE == kModePCG;
@@ -122,11 +128,157 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
     }
   };
 
+  double pq = 0.0;
+  const int src_right = (xl + 1) % c.nl;       // lane holding x + XPL
+  const int src_left = (xl + c.nl - 1) % c.nl;  // lane holding x - 1
+  if constexpr (PLANE) {
+    T nodeU[R + 1][XPL][3];
+    T nodeLo[R][XPL][3];             // bottom-plane nodes of the owned rows (p.q)
+    T PL[R][XPL][3][4];              // in-plane modes of the bottom plane, per element column
+    T CM[R][XPL][3][4];              // modal forces on the bottom plane from the layer below
+    T accL[R + 1][XPL][3];
+    T aL[R][XPL], aU[R][XPL], aN[R][XPL];
+    const int slot_size = (c.W + 1) * 64 * 3 * XPL;
+    // in-plane modes of every element column of the rows this wave computes
+    auto plane_modes = [&](const T (&nd)[R + 1][XPL][3], T (&P)[R][XPL][3][4]) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if (j >= rows) break;
+        T nb[2][3];
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) nb[oy][d] = __shfl(nd[j + oy][0][d], src_right, 64);
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            plane_forward(nd[j][xi][d], xi + 1 < XPL ? nd[j][xi + 1][d] : nb[0][d], nd[j + 1][xi][d],
+                          xi + 1 < XPL ? nd[j + 1][xi + 1][d] : nb[1][d], P[j][xi][d]);
+      }
+    };
+    Rows<MODE, R, XPL, T> raw;
+    fetch(wrapi(z0 - 1, nz), raw, aL);
+    settle(wrapi(z0 - 1, nz), raw, nodeU, false);
+    plane_modes(nodeU, PL);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          nodeLo[j][xi][d] = nodeU[j][xi][d];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) CM[j][xi][d][m] = T(0);
+        }
+    fetch(z0, raw, aU);
+    for (int li = 0; li <= c.ZC; ++li) {
+      const int zl = wrapi(z0 - 1 + li, nz), zu = wrapi(z0 + li, nz);
+      settle(zu, raw, nodeU, li < c.ZC);
+      if (li < c.ZC) fetch(wrapi(z0 + li + 1, nz), raw, aN);  // prefetch the next plane
+#pragma unroll
+      for (int j = 0; j <= R; ++j)
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) accL[j][xi][d] = T(0);
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if (j >= rows) break;
+        T nb[2][3];
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) nb[oy][d] = __shfl(nodeU[j + oy][0][d], src_right, 64);
+        T spill[2][3];  // contributions of this lane's last element column to x + 1 (next lane)
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi) {
+          T PU[3][4], Um[3][8], F[3][8];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            plane_forward(nodeU[j][xi][d], xi + 1 < XPL ? nodeU[j][xi + 1][d] : nb[0][d], nodeU[j + 1][xi][d],
+                          xi + 1 < XPL ? nodeU[j + 1][xi + 1][d] : nb[1][d], PU[d]);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              Um[d][m] = PL[j][xi][d][m] + PU[d][m];
+              Um[d][m | 4] = PU[d][m] - PL[j][xi][d][m];
+            }
+          }
+          hex8_modal_core<CUBIC, T>(Um, aL[j][xi], A.coef, F);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            T M[4], f[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              M[m] = CM[j][xi][d][m] + (F[d][m] - F[d][m | 4]);
+              CM[j][xi][d][m] = F[d][m] + F[d][m | 4];
+              PL[j][xi][d][m] = PU[d][m];
+            }
+            plane_transposed(M, f);
+            accL[j][xi][d] += f[0];
+            accL[j + 1][xi][d] += f[2];
+            if (xi + 1 < XPL) {
+              accL[j][xi + 1][d] += f[1];
+              accL[j + 1][xi + 1][d] += f[3];
+            } else {
+              spill[0][d] = f[1];
+              spill[1][d] = f[3];
+            }
+          }
+        }
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) accL[j + oy][0][d] += __shfl(spill[oy][d], src_left, 64);
+      }
+      // Export the contributions to the first node row of the next wave.
+      T* slots = lds + (li & 1) * slot_size;
+      T* slot = slots + wave * 64 * 3 * XPL;
+#pragma unroll
+      for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) slot[(xi * 3 + d) * 64 + lane] = halo ? accL[1][xi][d] : accL[R][xi][d];
+      __syncthreads();
+      if (!halo) {
+        const T* from = slots + (wave == 0 ? W : wave - 1) * 64 * 3 * XPL;
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) accL[0][xi][d] += from[(xi * 3 + d) * 64 + lane];
+        if (li >= 1 && lane_on) {
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            const unsigned rb = unsigned(zl * ny + ybase + j) * unsigned(nx);
+#pragma unroll
+            for (int xi = 0; xi < XPL; ++xi)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                st(out, at(rb + xo + xi, d), accL[j][xi][d]);
+                if (MODE == kModePCG) pq = fma(double(nodeLo[j][xi][d]), double(accL[j][xi][d]), pq);
+              }
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int xi = 0; xi < XPL; ++xi) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) nodeLo[j][xi][d] = nodeU[j][xi][d];
+          aL[j][xi] = aU[j][xi];
+          aU[j][xi] = aN[j][xi];
+        }
+    }
+    if (MODE == kModePCG) {
+      const double total = block_sum(pq, reinterpret_cast<double*>(lds + 2 * slot_size));
+      if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
+    }
+    return;
+  } else {
   T nodeL[R + 1][XPL][3], nodeU[R + 1][XPL][3];
   T carry[R][XPL][3];
   T accL[R + 1][XPL][3], accU[R + 1][XPL][3];
   T aL[R][XPL], aU[R][XPL], aN[R][XPL];
-  double pq = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j)
 #pragma unroll
@@ -134,8 +286,6 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) carry[j][xi][d] = T(0);
 
-  const int src_right = (xl + 1) % c.nl;       // lane holding x + XPL
-  const int src_left = (xl + c.nl - 1) % c.nl;  // lane holding x - 1
   const int slot_size = (c.W + 1) * 64 * 6 * XPL;
 
   Rows<MODE, R, XPL, T> raw;
@@ -267,6 +417,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   if (MODE == kModePCG) {
     const double total = block_sum(pq, reinterpret_cast<double*>(lds + 2 * slot_size));
     if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
+  }
   }
 }
 
@@ -623,15 +774,15 @@ int check_grid(const pgj_grid* g) {
   return PGJ_OK;
 }
 
-template <int MODE, int XPL, int R, int MAXT, typename T>
+template <int MODE, int XPL, int R, int MAXT, bool PLANE, typename T>
 int launch_rowmarch_t(const pgj_grid& g, int L, RowArgsT<T>& A, hipStream_t s) {
   const RowMarch& c = A.cfg;
   dim3 grid(c.by, c.bz, L), block(64 * (c.W + 1));
-  const size_t lds = 2 * size_t(c.W + 1) * 64 * 6 * XPL * sizeof(T) + 64 * 8;
+  const size_t lds = 2 * size_t(c.W + 1) * 64 * (PLANE ? 3 : 6) * XPL * sizeof(T) + 64 * 8;
   if (cubic(g))
-    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true, MAXT, T>), grid, block, lds, s, A);
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, true, MAXT, T, PLANE>), grid, block, lds, s, A);
   else
-    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false, MAXT, T>), grid, block, lds, s, A);
+    hipLaunchKernelGGL((hex8_rowmarch<MODE, XPL, R, false, MAXT, T, PLANE>), grid, block, lds, s, A);
   PGJ_LAUNCHED("hex8_rowmarch");
   return PGJ_OK;
 }
@@ -639,16 +790,21 @@ int launch_rowmarch_t(const pgj_grid& g, int L, RowArgsT<T>& A, hipStream_t s) {
 template <int MODE, typename T>
 int launch_rowmarch_m(const pgj_grid& g, int L, RowArgsT<T>& A, hipStream_t s) {
   const RowMarch& c = A.cfg;
-  if (c.R == 2) return launch_rowmarch_t<MODE, 1, 2, 320>(g, L, A, s);
-  if (c.xpl == 1) return launch_rowmarch_t<MODE, 1, 1, 576>(g, L, A, s);
-  return launch_rowmarch_t<MODE, 2, 1, 320>(g, L, A, s);
+  constexpr int kPlaneThreads = sizeof(T) == 8 ? 320 : 576;
+  if (c.plane) {
+    if (c.xpl == 1) return launch_rowmarch_t<MODE, 1, 1, kPlaneThreads, true>(g, L, A, s);
+    return launch_rowmarch_t<MODE, 2, 1, 320, true>(g, L, A, s);
+  }
+  if (c.R == 2) return launch_rowmarch_t<MODE, 1, 2, 320, false>(g, L, A, s);
+  if (c.xpl == 1) return launch_rowmarch_t<MODE, 1, 1, 576, false>(g, L, A, s);
+  return launch_rowmarch_t<MODE, 2, 1, 320, false>(g, L, A, s);
 }
 
 template <typename T>
 int launch_rowmarch(const pgj_grid& g, int L, int mode, RowArgsT<T>& A, hipStream_t s) {
   A.nz = g.nz; A.ny = g.ny; A.nx = g.nx;
   A.N = int64_t(g.nx) * g.ny * g.nz;
-  A.cfg = rowmarch_config(g, L);
+  A.cfg = rowmarch_config(g, L, int(sizeof(T)));
   A.coef = coef_of(g);
   return mode == kModeApply ? launch_rowmarch_m<kModeApply>(g, L, A, s) : launch_rowmarch_m<kModePCG>(g, L, A, s);
 }

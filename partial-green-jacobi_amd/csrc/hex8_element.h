@@ -32,34 +32,37 @@ struct ElementCoef {
   double vol;       // hx*hy*hz
 };
 
-// f = a * Ke(C0) * u for one element. u and f are 24 values, dof 3a+d; T is
-// the arithmetic precision (the coefficients are rounded to it).
+// In-plane forward butterflies of one component on a 2x2 node block
+// (n = x + 2y): x then y, (lo, hi) -> (lo + hi, hi - lo). P[m] for m = mx + 2 my.
+template <typename T>
+PGJ_HD inline void plane_forward(T n0, T n1, T n2, T n3, T (&P)[4]) {
+  const T a0 = n0 + n1, a1 = n1 - n0, b0 = n2 + n3, b1 = n3 - n2;
+  P[0] = a0 + b0;
+  P[2] = b0 - a0;
+  P[1] = a1 + b1;
+  P[3] = b1 - a1;
+}
+
+// In-plane transposed butterflies of one component: y then x, (lo, hi) -> (lo - hi, lo + hi).
+template <typename T>
+PGJ_HD inline void plane_transposed(const T (&M)[4], T (&f)[4]) {
+  const T t0 = M[0] - M[2], t2 = M[0] + M[2], t1 = M[1] - M[3], t3 = M[1] + M[3];
+  f[0] = t0 - t1;
+  f[1] = t0 + t1;
+  f[2] = t2 - t3;
+  f[3] = t2 + t3;
+}
+
+// Modal core: forward-transformed nodal values U[d][m] (m = mx + 2 my + 4 mz)
+// -> modal forces F[d][m] of a * Ke(C0), before the transposed butterflies.
 template <bool kCubic, typename T = double>
-PGJ_HD inline void hex8_element(const T (&u)[24], T a, const ElementCoef& c, T (&f)[24]) {
-  T U[3][8];
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-#pragma unroll
-    for (int n = 0; n < 8; ++n) U[d][n] = u[3 * n + d];
-  // Forward butterflies: (lo, hi) -> (lo + hi, hi - lo) along x, y, z.
-#pragma unroll
-  for (int b = 1; b < 8; b <<= 1)
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int n = 0; n < 8; ++n)
-        if (!(n & b)) {
-          const T lo = U[d][n], hi = U[d][n | b];
-          U[d][n] = lo + hi;
-          U[d][n | b] = hi - lo;
-        }
+PGJ_HD inline void hex8_modal_core(const T (&U)[3][8], T a, const ElementCoef& c, T (&F)[3][8]) {
   // Gradient scale: with a cubic voxel g^2 is folded into k_t.
   const T gs0 = kCubic ? T(1) : T(c.g[0]);
   const T gs1 = kCubic ? T(1) : T(c.g[1]);
   const T gs2 = kCubic ? T(1) : T(c.g[2]);
   const T gs[3] = {gs0, gs1, gs2};
   const T base = kCubic ? T(double(a) * c.vol * c.g[0] * c.g[0]) : T(double(a) * c.vol);
-  T F[3][8];
 #pragma unroll
   for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -98,6 +101,31 @@ PGJ_HD inline void hex8_element(const T (&u)[24], T a, const ElementCoef& c, T (
         }
       }
   }
+}
+
+// f = a * Ke(C0) * u for one element. u and f are 24 values, dof 3a+d; T is
+// the arithmetic precision (the coefficients are rounded to it).
+template <bool kCubic, typename T = double>
+PGJ_HD inline void hex8_element(const T (&u)[24], T a, const ElementCoef& c, T (&f)[24]) {
+  T U[3][8];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) U[d][n] = u[3 * n + d];
+  // Forward butterflies: (lo, hi) -> (lo + hi, hi - lo) along x, y, z.
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        if (!(n & b)) {
+          const T lo = U[d][n], hi = U[d][n | b];
+          U[d][n] = lo + hi;
+          U[d][n | b] = hi - lo;
+        }
+  T F[3][8];
+  hex8_modal_core<kCubic, T>(U, a, c, F);
   // Transposed butterflies: (lo, hi) -> (lo - hi, lo + hi) along z, y, x.
 #pragma unroll
   for (int b = 4; b >= 1; b >>= 1)

@@ -25,35 +25,39 @@ struct RowMarch {
   int valid;
   int xpl, nl;      // x per lane, active lanes = nx / xpl
   int R, W, ZC;
+  int plane;        // 1: plane-carry march (shared in-plane butterflies), 0: node-carry march
   int by, bz;       // blocks along y and z per load
 };
 
-// Tuned row-march shapes, keyed by (nz, ny, nx, L); set by pgj_hex8_set_shape
-// (the autotuner) and consulted by every launch. Returns false if unset.
-bool tuned_shape(const pgj_grid& g, int L, int* R, int* W, int* ZC);
+// Tuned row-march shapes, keyed by (nz, ny, nx, L, element bytes); set by
+// pgj_hex8_set_shape (the autotuner) and consulted by every launch.
+// Returns false if unset.
+bool tuned_shape(const pgj_grid& g, int L, int es, int* R, int* W, int* ZC, int* plane);
 
 // Shape preferences (R, W, ZC) for L loads per launch. ZC (planes marched by
 // one block) is the largest of 16, 8, 4, 2, 1 dividing nz that still gives
 // at least 192 blocks: longer marches re-read fewer halo planes, but the
 // launch must keep most of the 256 CUs busy. A tuned shape replaces the
 // preferences.
-inline RowMarch rowmarch_config(const pgj_grid& g, int L) {
+inline RowMarch rowmarch_config(const pgj_grid& g, int L, int es = 8) {
   RowMarch c{};
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   c.xpl = (nx + 63) / 64;
   if (c.xpl > 2 || nx % c.xpl != 0 || ny < 2 || nz < 2 || L < 1) return c;
   c.nl = nx / c.xpl;
-  int want_r = 1, want_w = c.xpl == 1 ? 8 : 4, want_z = 0;
+  int want_r = 1, want_w = c.xpl == 1 ? 8 : 4, want_z = 0, want_plane = 0;
   {
-    int r = 0, w = 0, z = 0;
-    if (tuned_shape(g, L, &r, &w, &z)) {
-      want_r = r; want_w = w; want_z = z;
+    int r = 0, w = 0, z = 0, p = 0;
+    if (tuned_shape(g, L, es, &r, &w, &z, &p)) {
+      want_r = r; want_w = w; want_z = z; want_plane = p;
     }
   }
-  c.R = (want_r == 2 && ny % 2 == 0 && c.xpl == 1) ? 2 : 1;  // two rows of two x per lane exceed 256 VGPRs
+  c.plane = want_plane ? 1 : 0;
+  // two rows of two x per lane, and two rows of the plane-carry march, exceed 256 VGPRs
+  c.R = (want_r == 2 && ny % 2 == 0 && c.xpl == 1 && !c.plane) ? 2 : 1;
   // block = W row-waves + 1 halo wave: at most 5 waves (320 threads, the launch
-  // bound of the R = 2 and two-x-per-lane kernels) or 9 waves (576) otherwise
-  const int wcap = (c.R == 2 || c.xpl == 2) ? 4 : 8;
+  // bound of the R = 2, two-x-per-lane and FP64 plane-carry kernels) or 9 waves (576)
+  const int wcap = (c.R == 2 || c.xpl == 2 || (c.plane && es == 8)) ? 4 : 8;
   const int wmax = want_w < wcap ? want_w : wcap;
   const int rows = ny / c.R;
   c.W = 1;
@@ -73,8 +77,8 @@ inline RowMarch rowmarch_config(const pgj_grid& g, int L) {
   return c;
 }
 
-inline int hex8_blocks_per_load(const pgj_grid& g, int L) {
-  RowMarch c = rowmarch_config(g, L);
+inline int hex8_blocks_per_load(const pgj_grid& g, int L, int es = 8) {
+  RowMarch c = rowmarch_config(g, L, es);
   if (c.valid) return c.by * c.bz;
   const int64_t n = int64_t(g.nx) * g.ny * g.nz;
   return int((n + 255) / 256);

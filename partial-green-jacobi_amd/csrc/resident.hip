@@ -194,7 +194,7 @@ int resident_init_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgs
   A.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
   A.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
   A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z + sl.vec();
-  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, nl); A.ppq_stride = hex8_blocks_bound(*g);
+  A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, nl, int(sizeof(T))); A.ppq_stride = hex8_blocks_bound(*g);
   A.st = state_view(state, L, l0);
   A.soa = 1;
   A.group = group;
@@ -221,7 +221,7 @@ int resident_stage_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArg
   F.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
   F.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
   F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z + v;
-  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl); F.ppq_stride = hex8_blocks_bound(*g);
+  F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl, int(sizeof(T))); F.ppq_stride = hex8_blocks_bound(*g);
   F.st = st;
   F.soa = 1;
   F.group = group;

@@ -39,8 +39,8 @@ SIGNATURES = {
     'pgj_hex8_avg_stress': (c_int, [GP, c_void_p, c_void_p, c_dp, c_void_p, c_void_p, c_void_p]),
     'pgj_scale_ws_bytes': (c_size_t, [GP]),
     'pgj_diagonal_scale': (c_int, [GP, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
-    'pgj_hex8_set_shape': (c_int, [GP, c_int, c_int, c_int, c_int]),
-    'pgj_hex8_get_shape': (c_int, [GP, c_int, ctypes.POINTER(c_int)]),
+    'pgj_hex8_set_shape': (c_int, [GP, c_int, c_int, c_int, c_int, c_int, c_int]),
+    'pgj_hex8_get_shape': (c_int, [GP, c_int, c_int, ctypes.POINTER(c_int)]),
     'pgj_aos_to_soa': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p]),
     'pgj_soa_to_aos': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p]),
     'pgj_resident_scale': (c_int, [GP, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -131,8 +131,9 @@ class DesignGroup:
         return out.reshape(6, 6)
 
     def shape_info(self):
-        out = (_lib.c_int * 4)()
-        _lib.check(_lib.lib().pgj_hex8_get_shape(self.op.grid, self.L, out))
+        """(R, W, ZC, plane, blocks per load) of the row-march launch of this set."""
+        out = (_lib.c_int * 5)()
+        _lib.check(_lib.lib().pgj_hex8_get_shape(self.op.grid, self.L, 8, out))
         return tuple(out)
 
 
@@ -160,7 +161,9 @@ class BatchedDesigns:
             grid = g.op.grid
             lib = _lib.lib()
             if candidates is None:
-                cand = [(1, w, zc) for w in (2, 4, 8) for zc in (1, 2, 4, 8, 16) if n % zc == 0]
+                # (R, W, ZC, plane); the FP64 plane-carry march runs at most 4 row waves per block
+                cand = [(1, w, zc, p) for p in (0, 1) for w in (2, 4, 8) for zc in (1, 2, 4, 8, 16)
+                        if n % zc == 0 and not (p and w > 4)]
             else:
                 cand = list(candidates)
             g.setup(1e-8)
@@ -170,7 +173,7 @@ class BatchedDesigns:
             g.stage(0, g.p0, g.p1)
             best = None
             for shape in cand:
-                _lib.check(lib.pgj_hex8_set_shape(grid, g.L, *shape))
+                _lib.check(lib.pgj_hex8_set_shape(grid, g.L, 8, *shape))
                 g.stage(0, g.p0, g.p1)   # warm
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
@@ -181,7 +184,7 @@ class BatchedDesigns:
                 t = e0.elapsed_time(e1) / reps
                 if best is None or t < best[0]:
                     best = (t, shape)
-            _lib.check(lib.pgj_hex8_set_shape(grid, g.L, *best[1]))
+            _lib.check(lib.pgj_hex8_set_shape(grid, g.L, 8, *best[1]))
             chosen[n] = dict(shape=best[1], ms=best[0])
             g.u.zero_()
         torch.cuda.synchronize()

@@ -96,3 +96,25 @@ def test_operator_properties_at_64():
     assert tr < 1e-13
     ref = hex8.apply(a.cpu().numpy(), u[:1].cpu().numpy(), C0)
     assert rel(Ku[:1].cpu().numpy(), ref) < 1e-14
+
+
+@pytest.mark.parametrize('grid,voxel', GRIDS, ids=[str(g) for g, _ in GRIDS])
+@pytest.mark.parametrize('zc,w', [(1, 2), (2, 4), (4, 1)])
+def test_plane_carry_march_matches_oracle(grid, voxel, zc, w):
+    """The plane-carry row-march variant (shared in-plane butterflies) against the oracle, several shapes."""
+    rng, a, op, vox = make(grid, voxel, 99)
+    u = rng.normal(size=(3, 3 * a.size))
+    ref = hex8.apply(a, u, C0, vox)
+    lib = _lib.lib()
+    _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 1, w, min(zc, grid[0]), 1))
+    try:
+        shape = (_lib.c_int * 5)()
+        _lib.check(lib.pgj_hex8_get_shape(op.grid, 3, 8, shape))
+        assert shape[3] == 1
+        ut = torch.tensor(u, device='cuda')
+        got = op.apply_batched(ut).cpu().numpy()
+        assert rel(got, ref) < 1e-14
+        masked = op.apply_batched(ut, torch.tensor([False, True, False], device='cuda')).cpu().numpy()
+        assert (masked[0] == 0).all() and np.array_equal(masked[1], got[1])
+    finally:
+        _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 0, 0, 0, 0))

@@ -123,3 +123,26 @@ def test_resident_large_certifies(n):
         f = hex8.rhs(a, np.eye(6)[m], C0)
         res = np.linalg.norm(f - hex8.apply(a, u[l:l + 1], C0)) / np.linalg.norm(f)
         assert res <= 1.0001e-8, (n, m, res)
+
+
+@pytest.mark.parametrize('n', [16, 32])
+def test_plane_carry_march_in_pcg(n):
+    """The resident iteration with the plane-carry HEX8 variant matches the oracle recurrence."""
+    from pgj import _lib
+    a = density(n, 1e-4, 2)
+    r = pgj.Resident(n, load_mode='parallel', batch=6)
+    lib = _lib.lib()
+    _lib.check(lib.pgj_hex8_set_shape(r.op.grid, 6, 8, 1, 4, 4, 1))
+    try:
+        r.capture()
+        r.material(torch.tensor(a, device='cuda'))
+        r.select('0.5')
+        infos, _ = r.solve(rtol=1e-10)
+        ref = pcg.ResidentPCG(a, C0, np.eye(6), batch=6, s=green.scale(a, .5, C0))
+        rinfo, _ = ref.solve(rtol=1e-10)
+        for gi, oi in zip(infos, rinfo):
+            assert gi['converged'] and abs(gi['iterations'] - oi['iterations']) <= 1
+        H = r.effective_tensor().cpu().numpy()
+        assert np.abs(H - design.effective_tensor(a, ref.solutions, C0)).max() / np.abs(H).max() < 1e-10
+    finally:
+        _lib.check(lib.pgj_hex8_set_shape(r.op.grid, 6, 8, 0, 0, 0, 0))
