@@ -24,6 +24,10 @@ import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each stage at 64^3 with 6 loads, from the FETCH_SIZE / WRITE_SIZE counter runs of
+# `bench.py --profile-stages` scaled by a 256 MiB copy (read factor 2.0, write factor 1.0);
+# profiles/pcg_stages_64cube_6loads_traffic.md holds the commands and the table.
+MEASURED_TRAFFIC_64_6 = {'hex8_rowmarch': 184.8e6, 'fxy': 276.5e6, 'fz': 120.3e6, 'fxyi': 83.5e6}
 
 
 def parse():
@@ -405,7 +409,8 @@ def main():
         'max_relative_residual': max(r['rel'] for r in timed),
         'ch_stress_vs_energy_rel': ch_err,
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None},
+                     'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': MEASURED_TRAFFIC_64_6[dom] if (n, res.batch) == (64, 6) else None},
         'stages': stages, 'probe_active': active,
         'cpu_baseline': cpu,
     }

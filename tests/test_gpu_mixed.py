@@ -51,8 +51,8 @@ def test_mixed_matches_fp64(n, emin, radius, alpha):
 
 
 def test_mixed_matches_oracle_and_designs_are_independent():
-    n = 8
-    fields = [density(n, 1e-4, 1, seed) for seed in (3, 4)]
+    n = 16
+    fields = [density(n, 1e-4, 2, seed) for seed in (3, 4)]
     both = MixedResident(n, designs=2).capture()
     for d, a in enumerate(fields):
         both.material(d, torch.tensor(a, device='cuda'))
