@@ -59,6 +59,8 @@ def parse():
     p.add_argument('--serve-rate', type=float, default=20.0)
     p.add_argument('--serve-clients', type=int, default=64)
     p.add_argument('--serve-sizes', default='32,48,64,96,128')
+    p.add_argument('--serve-precision', choices=('mixed', 'fp64'), default='mixed',
+                   help='FP32 inner PCG inside FP64 refinement (certified 1e-8) or the plain FP64 solve per slot')
     p.add_argument('--batched-per-size', type=int, default=12)
     return p.parse_args()
 
@@ -267,6 +269,7 @@ def mixed_form(args):
         }
         del plain, mixed
         torch.cuda.empty_cache()
+    steps = design_steps_compare(args)
     r64 = rows['64']
     out = {
         'metric': 'certified 6-load solves/s, mixed-precision form (FP32 inner PCG, FP64 refinement to 1e-8)',
@@ -276,8 +279,64 @@ def mixed_form(args):
         'config': {'workload': '64^3 (Emin 1e-6) and 128^3 (Emin 1e-9), radius-4 filtered noise, alpha '
                                f'{args.alpha:g}, 6 unit-strain loads, cold start, inner tol 1e-4, certified 1e-8'},
         'cases': rows,
+        'design_steps_64': steps,
     }
     print(json.dumps(out), flush=True)
+
+
+def design_steps_compare(args):
+    """The default design-step workload (64^3, Emin 1e-6, warm starts, OC) with plain FP64 and mixed solves."""
+    import pgj
+    from pgj.density import filtered_noise
+    from pgj.mixed import MixedResident
+    from pgj.outer import PaperOuter
+
+    n = 64
+    rho0 = torch.tensor(filtered_noise(n, args.seed, args.radius), device='cuda')
+    out = {}
+    for kind in ('fp64', 'mixed'):
+        outer = PaperOuter(n, args.volume, rmin=args.rmin, beta=4.0, eta=0.5, emin=args.emin,
+                           move=args.move).capture()
+        outer.set_initial(rho0)
+        if kind == 'fp64':
+            res = pgj.Resident(n, np.eye(6), 'parallel', batch=6).capture()
+        else:
+            res = MixedResident(n).capture()
+        lat, its, chs = [], [], []
+        for k in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            outer.evaluate()
+            if kind == 'fp64':
+                res.material(outer.a)
+                res.select(args.alpha)
+                infos, _ = res.solve(rtol=1e-8)
+                _, local, _ = res.energy('shear')
+                H = res.effective_tensor()
+            else:
+                res.material(0, outer.a)
+                res.select(0, args.alpha)
+                infos, _ = res.solve(rtol=1e-8)
+                local = -res.energies(0)[3:].sum(0)
+                H = res.effective_tensor(0)
+            outer.step(local)
+            outer.advance()
+            torch.cuda.synchronize()
+            if k >= args.warmup:
+                lat.append(time.perf_counter() - t0)
+                its.append(sum(i['iterations'] for i in infos))
+                chs.append(H.cpu().numpy())
+        out[kind] = {'steps_per_s': len(lat) / sum(lat), 'p50_ms': float(np.percentile(lat, 50) * 1e3),
+                     'p99_ms': float(np.percentile(lat, 99) * 1e3), 'iterations_per_step': its}
+        out[kind + '_CH_last'] = chs[-1]
+        del res, outer
+        torch.cuda.empty_cache()
+    a, b = out.pop('fp64_CH_last'), out.pop('mixed_CH_last')
+    out['speedup'] = out['mixed']['steps_per_s'] / out['fp64']['steps_per_s']
+    # the two runs follow the same OC trajectory only approximately (solutions differ at 1e-8), so this is the
+    # relative C^H gap after the last step, not a parity figure
+    out['ch_gap_last_step'] = float(np.abs(a - b).max() / np.abs(a).max())
+    return out
 
 
 def serve_form(args):
@@ -289,7 +348,7 @@ def serve_form(args):
     arrivals = poisson_arrivals(args.serve_rate, args.serve_seconds, [c.steps for c in clients], args.seed)
     free, total = torch.cuda.mem_get_info()
     budget = 0.6 * free
-    server = Server(sizes, budget, max_slots=16)
+    server = Server(sizes, budget, max_slots=16, precision=args.serve_precision)
     done = server.run(clients, arrivals, horizon=args.serve_seconds, deadline=2 * args.serve_seconds)
     elapsed = server.elapsed
     stats = latency_stats(done)
@@ -301,10 +360,12 @@ def serve_form(args):
         'metric': 'design-step requests/s through the continuous-batching server, with p50/p99 latency',
         'value': len(done) / elapsed, 'unit': 'certified design steps/s', 'n_gpus': 1, 'steps': len(done),
         'warmup': 0, 'ms_per_step': 1e3 * elapsed / max(len(done), 1), 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic (seeded clients)',
+        'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 inner / f64 outer' if args.serve_precision == 'mixed' else 'f64',
+        'data': 'synthetic (seeded clients)',
         'config': {'workload': f'{len(clients)} clients, grids {sizes}, Emin in (1e-4, 1e-6, 1e-9), alpha U[0,1], '
                                f'Poisson arrivals {args.serve_rate:g}/s for {args.serve_seconds:g} s, 6 loads, '
-                               f'FP64 certified at 1e-8', 'arrivals': len(arrivals)},
+                               f'{args.serve_precision} solves certified in FP64 at 1e-8', 'arrivals': len(arrivals)},
         'latency_ms': {'p50': 1e3 * stats['p50'], 'p99': 1e3 * stats['p99'], 'mean': 1e3 * stats['mean']},
         'latency_ms_by_size': {str(n): {'p50': 1e3 * float(np.percentile(v, 50)),
                                         'p99': 1e3 * float(np.percentile(v, 99)), 'count': len(v)}

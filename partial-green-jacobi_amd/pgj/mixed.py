@@ -88,34 +88,56 @@ class MixedResident:
         _lib.call('pgj_to_f32', 2 * self.scale_hat.shape[1], _lib.ptr(self.scale_hat[d]),
                   _lib.ptr(self.scale_hat32[d]), _lib.stream())
 
-    # -- launches ---------------------------------------------------------------------
-    def setup(self, rtol):
-        m = np.tile(self.loads, (self.D, 1)).reshape(-1)
-        _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.a), _lib.host_doubles(m),
+    # -- launches: designs [d0, d0 + nd) (default: all) --------------------------------
+    def _slice(self, d0, nd):
+        nd = self.D - d0 if nd is None else nd
+        if d0 < 0 or nd < 1 or d0 + nd > self.D:
+            raise ValueError(f'design range [{d0}, {d0 + nd}) outside 0..{self.D}')
+        return d0 * self.G, nd * self.G
+
+    def setup(self, rtol, d0=0, nd=None):
+        l0, nl = self._slice(d0, nd)
+        m = np.tile(self.loads, (nl // self.G, 1)).reshape(-1)
+        _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.a), _lib.host_doubles(m),
                   float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
 
-    def certify(self):
-        _lib.call('pgj_resident_certify', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.a), _lib.ptr(self.u),
+    def certify(self, d0=0, nd=None):
+        l0, nl = self._slice(d0, nd)
+        _lib.call('pgj_resident_certify', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.a), _lib.ptr(self.u),
                   _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
 
-    def begin(self):
-        _lib.call('pgj_inner_begin', self.op.grid, self.L, 0, self.L, _lib.ptr(self.r), _lib.ptr(self.state),
+    def begin(self, d0=0, nd=None):
+        """Scaled FP32 residual of the last certification, d = 0, inner masks and first direction."""
+        l0, nl = self._slice(d0, nd)
+        _lib.call('pgj_inner_begin', self.op.grid, self.L, l0, nl, _lib.ptr(self.r), _lib.ptr(self.state),
                   self.inner_rtol, _lib.ptr(self.r32), _lib.ptr(self.d), _lib.ptr(self.state32), _lib.ptr(self.ws32),
                   _lib.stream())
-        _lib.call('pgj_inner_init', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.plan32),
+        _lib.call('pgj_inner_init', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.plan32),
                   _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.r32), _lib.ptr(self.z),
                   _lib.ptr(self.state32), _lib.ptr(self.ws32), _lib.stream())
 
     def iteration(self):
+        """``chunk`` FP32 inner iterations of every load of the set (stopped loads exit at once)."""
         _lib.call('pgj_inner_iterate', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.plan32),
                   _lib.ptr(self.a32), _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.d),
                   _lib.ptr(self.r32), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.z),
                   _lib.ptr(self.state32), _lib.ptr(self.ws32), self.chunk, _lib.stream())
 
-    def end_and_certify(self):
-        _lib.call('pgj_inner_end', self.op.grid, self.L, 0, self.L, _lib.ptr(self.d), _lib.ptr(self.ws32),
+    def end(self, d0=0, nd=None):
+        """u += |r| d in FP64 for the designs' loads."""
+        l0, nl = self._slice(d0, nd)
+        _lib.call('pgj_inner_end', self.op.grid, self.L, l0, nl, _lib.ptr(self.d), _lib.ptr(self.ws32),
                   _lib.ptr(self.u), _lib.stream())
+
+    def end_and_certify(self):
+        self.end()
         self.certify()
+
+    def status(self, d0=0, nd=None):
+        """Host copy of (rhs norm, |r|, threshold) per load of designs [d0, d0 + nd) from the last certification."""
+        l0, nl = self._slice(d0, nd)
+        sl = slice(l0, l0 + nl)
+        return torch.stack((self.norm[sl], self.absolute[sl], self.target2[sl].sqrt()), dim=1).cpu().numpy()
 
     def capture(self):
         self.setup(1e-8)

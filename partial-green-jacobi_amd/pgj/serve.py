@@ -33,19 +33,28 @@ import torch
 from . import _lib
 from .batched import DesignGroup
 from .density import filtered_noise
+from .mixed import MixedResident
 from .outer import PaperOuter
 
 LOADS6 = np.eye(6)
 
 
-def slot_bytes(n, loads=6):
-    """HBM bytes one design slot of grid n holds in a resident set (vectors, spectrum, fields, partials)."""
+def slot_bytes(n, loads=6, precision='fp64'):
+    """HBM bytes one design slot of grid n holds in a resident set (vectors, spectra, fields, partials).
+
+    ``precision='mixed'``: three FP64 outer vectors and six FP32 inner vectors, FP32 fields and spectrum copies.
+    """
     N = n ** 3
     nf = n * n * (n // 2 + 1)
+    partials = loads * (n * n + n * 6 + n * n * 4 + (N + 255) // 256 * 4) * 8
+    fields = N * 8 + 3 * N * 8 + 3 * nf * 16
+    if precision == 'mixed':
+        vectors = 3 * loads * 3 * N * 8 + 6 * loads * 3 * N * 4
+        spectrum = loads * 3 * nf * (16 + 8)
+        fields += N * 4 + 3 * N * 4 + 3 * nf * 8
+        return vectors + spectrum + fields + 2 * partials
     vectors = 7 * loads * 3 * N * 8
     spectrum = loads * 3 * nf * 16
-    fields = N * 8 + 3 * N * 8 + 3 * nf * 16
-    partials = loads * (n * n + n * 6 + n * n * 4 + (N + 255) // 256 * 4) * 8
     return vectors + spectrum + fields + partials
 
 
@@ -73,17 +82,23 @@ class Server:
     """Design-step server: ``sizes`` are the grid sizes served, ``budget`` the HBM bytes for resident sets."""
 
     def __init__(self, sizes, budget, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, chunk=4, max_slots=16,
-                 device='cuda', graphs=True):
+                 device='cuda', graphs=True, precision='fp64', inner_rtol=1e-4, max_outer=30):
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.sizes = sorted(int(n) for n in sizes)
-        self.rtol, self.chunk, self.device = rtol, chunk, device
+        self.rtol, self.chunk, self.device, self.precision = rtol, chunk, device, precision
         self.volume, self.move, self.rmin = volume, move, rmin
+        self.max_outer = max_outer
         share = budget / len(self.sizes)
         self.groups = {}
         for n in self.sizes:
-            slots = int(min(max_slots, share // slot_bytes(n)))
+            slots = int(min(max_slots, share // slot_bytes(n, precision=precision)))
             if slots < 1:
                 raise ValueError(f'HBM budget {budget / 2**30:.1f} GiB holds no slot of {n}^3')
-            self.groups[n] = DesignGroup(n, slots, LOADS6, chunk, device=device)
+            if precision == 'mixed':
+                self.groups[n] = MixedResident(n, LOADS6, slots, inner_rtol, chunk, max_outer, device=device)
+            else:
+                self.groups[n] = DesignGroup(n, slots, LOADS6, chunk, device=device)
         self.free = {n: list(range(g.D))[::-1] for n, g in self.groups.items()}
         self.owner = {n: [None] * g.D for n, g in self.groups.items()}
         self.solving = {n: {} for n in self.sizes}          # slot -> Request
@@ -96,9 +111,14 @@ class Server:
     def _capture(self):
         for n, g in self.groups.items():
             g.setup(self.rtol)
-            g.r.normal_()
-            g.certify()
-            g.initialize()
+            if self.precision == 'mixed':
+                g.u.normal_()
+                g.certify()
+                g.begin()
+            else:
+                g.r.normal_()
+                g.certify()
+                g.initialize()
             g.iteration()
             torch.cuda.synchronize()
             graph = torch.cuda.CUDAGraph()
@@ -107,7 +127,7 @@ class Server:
             self.graphs[n] = graph
             torch.cuda.synchronize()
             g.u.zero_()
-            g.active.zero_()
+            (g.active32 if self.precision == 'mixed' else g.active).zero_()
         torch.cuda.synchronize()
 
     def _worker(self, n, emin):
@@ -120,7 +140,7 @@ class Server:
         return w
 
     def memory(self):
-        return {n: g.D * slot_bytes(n) for n, g in self.groups.items()}
+        return {n: g.D * slot_bytes(n, precision=self.precision) for n, g in self.groups.items()}
 
     # -- request lifecycle --------------------------------------------------------------
     def _admit(self, client):
@@ -142,7 +162,11 @@ class Server:
         g.select(c.slot, c.alpha)
         g.setup(self.rtol, c.slot, 1)
         g.certify(c.slot, 1)
-        g.initialize(c.slot, 1)
+        if self.precision == 'mixed':
+            req.outer, req.inner = 0, np.zeros(6, dtype=np.int64)
+            req.outer_pending = True     # the host decides after the first certification
+        else:
+            g.initialize(c.slot, 1)
         self.solving[c.n][c.slot] = req
 
     def _advance(self):
@@ -158,7 +182,8 @@ class Server:
         sizes = [n for n in self.sizes if self.solving[n]]
         if not sizes:
             return []
-        act = torch.cat([self.groups[n].active for n in sizes]).cpu().numpy()
+        flag = (lambda g: g.active32) if self.precision == 'mixed' else (lambda g: g.active)
+        act = torch.cat([flag(self.groups[n]) for n in sizes]).cpu().numpy()
         out, off = [], 0
         for n in sizes:
             D = self.groups[n].D
@@ -167,10 +192,37 @@ class Server:
             out.extend((n, s) for s in self.solving[n] if not a[s].any())
         return out
 
+    def _complete_mixed(self, n, slot):
+        """One refinement boundary of a mixed-precision design: returns the (6, 5) status rows once certified."""
+        g = self.groups[n]
+        req = self.solving[n][slot]
+        if not req.outer_pending:
+            sl = slice(6 * slot, 6 * slot + 6)
+            if bool(g.bad32[sl].any().item()):
+                raise RuntimeError(('FP32 inner PCG broke down', req.client.cid))
+            req.inner += g.iterations32[sl].cpu().numpy()
+            g.end(slot, 1)
+            g.certify(slot, 1)
+        req.outer_pending = False
+        v = g.status(slot, 1)
+        ok = np.isfinite(v[:, 1]) & (v[:, 1] <= v[:, 2])
+        if ok.all():
+            return np.column_stack((v, req.inner.astype(np.float64), np.zeros(6)))
+        if req.outer >= self.max_outer or not np.all(np.isfinite(v[:, 1])):
+            raise RuntimeError(('mixed-precision refinement did not certify', req.client.cid, v.tolist()))
+        req.outer += 1
+        g.begin(slot, 1)
+        return None
+
     def _complete(self, n, slot, t_end):
         g = self.groups[n]
         req = self.solving[n][slot]
         c = req.client
+        if self.precision == 'mixed':
+            diag = self._complete_mixed(n, slot)
+            if diag is None:
+                return False
+            return self._finalize(n, slot, t_end, diag)
         g.certify(slot, 1)
         sl = slice(6 * slot, 6 * slot + 6)
         diag = torch.stack((g.norm[sl], g.absolute[sl], g.target2[sl].sqrt(), g.iterations[sl].to(torch.float64),
@@ -183,6 +235,13 @@ class Server:
             req.restarts += 1
             g.initialize(slot, 1)
             return False
+        return self._finalize(n, slot, t_end, diag)
+
+    def _finalize(self, n, slot, t_end, diag):
+        """C^H, energies, OC update and the response of a certified design step."""
+        g = self.groups[n]
+        req = self.solving[n][slot]
+        c = req.client
         CH = g.effective_tensor(slot)
         u = g.solutions(slot)
         e = torch.empty((6, g.N), dtype=torch.float64, device=u.device)
@@ -196,7 +255,8 @@ class Server:
         c.x.copy_(w.new)
         req.end = t_end
         c.history.append(dict(CH=CH.cpu().numpy(), iterations=diag[:, 3].astype(int).tolist(),
-                              restarts=req.restarts, bisections=count, change=float(w.change),
+                              restarts=req.restarts, outer=getattr(req, 'outer', 0), bisections=count,
+                              change=float(w.change),
                               latency=req.end - req.arrival, max_rel=float((diag[:, 1] / diag[:, 0]).max())))
         c.done += 1
         c.busy = False

@@ -25,21 +25,22 @@ def clients():
     return [Client(i, n, emin, alpha, seed, STEPS) for i, (n, emin, alpha, seed) in enumerate(SPECS)]
 
 
-def test_same_alone_and_in_mix():
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_same_alone_and_in_mix(precision):
     mix = clients()
     # staggered arrivals: client 1 and 2 join while client 0 is solving
     arrivals = sorted([(0.0 + 0.001 * k, 0) for k in range(STEPS)] + [(0.002 + 0.001 * k, 1) for k in range(STEPS)]
                       + [(0.004 + 0.001 * k, 2) for k in range(STEPS)])
-    print('serve mix', flush=True)
-    server = Server(SIZES, BUDGET, max_slots=4)
+    print(f'serve mix {precision}', flush=True)
+    server = Server(SIZES, BUDGET, max_slots=4, precision=precision)
     done = server.run(mix, arrivals)
     assert len(done) == 3 * STEPS and server.unserved == 0
     stats = latency_stats(done)
     assert stats['count'] == 9 and stats['p99'] >= stats['p50'] > 0
     for i in range(3):
         alone = clients()
-        print(f'serve alone {i}', flush=True)
-        solo = Server(SIZES, BUDGET, max_slots=4)
+        print(f'serve alone {i} {precision}', flush=True)
+        solo = Server(SIZES, BUDGET, max_slots=4, precision=precision)
         solo.run(alone, [(0.001 * k, i) for k in range(STEPS)])
         a, b = mix[i].history, alone[i].history
         assert len(a) == len(b) == STEPS
