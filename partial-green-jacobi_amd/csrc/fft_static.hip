@@ -59,33 +59,43 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
   double sq = 0.0, sm = 0.0;
   // two nodes (x even, x+1) per step: one packed complex of the x transform
   const R ra = R(alpha);
+  // distinct arrays: the loads of later steps may be issued ahead of the stores of earlier ones
+  const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
+  const X* __restrict__ rinp = reinterpret_cast<const X*>(A.rin + off);
+  X* __restrict__ up = reinterpret_cast<X*>(A.u + off);
+  X* __restrict__ rp = reinterpret_cast<X*>(A.r + off);
+  const X* __restrict__ pp_ = reinterpret_cast<const X*>(A.p + off);
+  const X* __restrict__ qp = reinterpret_cast<const X*>(A.q + off);
+  const bool scaled = A.s != nullptr;
+#pragma unroll 4
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
+    const int h = e / 2;
     X v;
     if (MODE == kFxyScale) {
-      v = *reinterpret_cast<const X*>(A.s + soff + e);
+      v = sp[h];
     } else if (MODE == kFxyPre) {
-      const X rv = *reinterpret_cast<const X*>(A.rin + off + e);
+      const X rv = rinp[h];
       sq = fma(double(rv.x), double(rv.x), fma(double(rv.y), double(rv.y), sq));
       sm += double(rv.x) + double(rv.y);
-      if (A.s) {
-        const X sv = *reinterpret_cast<const X*>(A.s + soff + e);
+      if (scaled) {
+        const X sv = sp[h];
         v = c2(sv.x * rv.x, sv.y * rv.y);
       } else {
         v = rv;
       }
     } else {
-      const X uu = *reinterpret_cast<const X*>(A.u + off + e);
-      const X rr = *reinterpret_cast<const X*>(A.r + off + e);
-      const X pp = *reinterpret_cast<const X*>(A.p + off + e);
-      const X qq = *reinterpret_cast<const X*>(A.q + off + e);
+      const X uu = up[h];
+      const X rr = rp[h];
+      const X pp = pp_[h];
+      const X qq = qp[h];
       const X un = c2(fma(ra, pp.x, uu.x), fma(ra, pp.y, uu.y));
       const X rn = c2(fma(-ra, qq.x, rr.x), fma(-ra, qq.y, rr.y));
-      *reinterpret_cast<X*>(A.u + off + e) = un;
-      *reinterpret_cast<X*>(A.r + off + e) = rn;
+      up[h] = un;
+      rp[h] = rn;
       sq = fma(double(rn.x), double(rn.x), fma(double(rn.y), double(rn.y), sq));
       sm += double(rn.x) + double(rn.y);
-      if (A.s) {
-        const X sv = *reinterpret_cast<const X*>(A.s + soff + e);
+      if (scaled) {
+        const X sv = sp[h];
         v = c2(sv.x * rn.x, sv.y * rn.y);
       } else {
         v = rn;
@@ -193,23 +203,31 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
     return;
   }
   double rz = 0.0, sw[3] = {0.0, 0.0, 0.0};
-  for (int e = threadIdx.x; e < N * KC; e += T) {
+  // a fixed number of frequency slots per thread, unrolled, so the symbol and
+  // s_hat loads of every slot are in flight together
+  constexpr int EPT = (N * KC + T - 1) / T;
+  const X* __restrict__ shat_d = A.shat ? A.shat + int64_t(design_of(l, A.group)) * 3 * C::NF : nullptr;
+  const R* __restrict__ sym = A.sym;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = threadIdx.x + k * T;
+    if ((N * KC) % T != 0 && e >= N * KC) break;
     const int kcol = e % KC, zz = e / KC;
     const int kx = kx0 + kcol;
     const int64_t fidx = (int64_t(zz) * N + y) * NKX + kx;
     X v[3], sh[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = cols[(c * N + zz) * KC + kcol];
-    if (A.shat) {
+    if (shat_d) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const R mc = R(mean[c]);
-        sh[c] = A.shat[(int64_t(design_of(l, A.group)) * 3 + c) * C::NF + fidx];
+        sh[c] = shat_d[int64_t(c) * C::NF + fidx];
         v[c] = c2(fma(-mc, sh[c].x, v[c].x), fma(-mc, sh[c].y, v[c].y));
       }
     }
-    const R g0 = A.sym[fidx], g1 = A.sym[C::NF + fidx], g2 = A.sym[2 * C::NF + fidx];
-    const R g3 = A.sym[3 * C::NF + fidx], g4 = A.sym[4 * C::NF + fidx], g5 = A.sym[5 * C::NF + fidx];
+    const R g0 = sym[fidx], g1 = sym[C::NF + fidx], g2 = sym[2 * C::NF + fidx];
+    const R g3 = sym[3 * C::NF + fidx], g4 = sym[4 * C::NF + fidx], g5 = sym[5 * C::NF + fidx];
     X w[3];
     w[0] = c2(g0 * v[0].x + g3 * v[1].x + g4 * v[2].x, g0 * v[0].y + g3 * v[1].y + g4 * v[2].y);
     w[1] = c2(g3 * v[0].x + g1 * v[1].x + g5 * v[2].x, g3 * v[0].y + g1 * v[1].y + g5 * v[2].y);
@@ -219,7 +237,7 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) t += double(v[c].x) * double(w[c].x) + double(v[c].y) * double(w[c].y);
     rz = fma(wgt, t, rz);
-    if (A.shat) {
+    if (shat_d) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         sw[c] = fma(wgt, double(sh[c].x) * double(w[c].x) + double(sh[c].y) * double(w[c].y), sw[c]);
@@ -291,16 +309,20 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
   sfft<true, C::M, typename C::Rows, T>(plane, A.twx, 2);
   const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
+  const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
+  X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
+  const bool scaled = A.s != nullptr;
+  const R m = R(mean);
+#pragma unroll 4
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
     const int x = e % N, y = e / N;
     const X w = plane[y * C::RS + x / 2];
     X v = w;
-    if (A.s) {
-      const X sv = *reinterpret_cast<const X*>(A.s + soff + e);
-      const R m = R(mean);
+    if (scaled) {
+      const X sv = sp[e / 2];
       v = c2(sv.x * w.x - m, sv.y * w.y - m);
     }
-    *reinterpret_cast<X*>(A.zout + off + e) = v;
+    zp[e / 2] = v;
   }
 }
 
