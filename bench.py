@@ -341,7 +341,7 @@ def design_steps_compare(args):
 
 def serve_form(args):
     """Poisson design-step requests from many clients through the continuous-batching server."""
-    from pgj.serve import Server, latency_stats, make_clients, poisson_arrivals
+    from pgj.serve import SequentialServer, Server, latency_stats, make_clients, poisson_arrivals
 
     sizes = [int(v) for v in args.serve_sizes.split(',')]
     clients = make_clients(args.serve_clients, sizes, (1e-4, 1e-6, 1e-9), 50, args.seed)
@@ -351,8 +351,18 @@ def serve_form(args):
     server = Server(sizes, budget, max_slots=16, precision=args.serve_precision)
     done = server.run(clients, arrivals, horizon=args.serve_seconds, deadline=2 * args.serve_seconds)
     elapsed = server.elapsed
+    server_unserved, server_slots = server.unserved, {str(n): g.D for n, g in server.groups.items()}
+    resident = sum(server.memory().values()) / 2 ** 30
+    del server
+    torch.cuda.empty_cache()
+    # the plain FP64 drop-in on the same request stream, one request at a time
+    base_clients = make_clients(args.serve_clients, sizes, (1e-4, 1e-6, 1e-9), 50, args.seed)
+    base = SequentialServer(sizes)
+    base_done = base.run(base_clients, arrivals, horizon=args.serve_seconds, deadline=2 * args.serve_seconds)
+    base_stats = latency_stats(base_done)
     stats = latency_stats(done)
     rel = max((h['max_rel'] for c in clients for h in c.history), default=0.0)
+    unserved, slots = server_unserved, server_slots
     by_size = {}
     for r in done:
         by_size.setdefault(r.client.n, []).append(r.end - r.arrival)
@@ -370,9 +380,11 @@ def serve_form(args):
         'latency_ms_by_size': {str(n): {'p50': 1e3 * float(np.percentile(v, 50)),
                                         'p99': 1e3 * float(np.percentile(v, 99)), 'count': len(v)}
                                for n, v in sorted(by_size.items())},
-        'unserved': server.unserved, 'max_relative_residual': rel,
-        'slots': {str(n): g.D for n, g in server.groups.items()},
-        'resident_gib': sum(server.memory().values()) / 2 ** 30,
+        'unserved': unserved, 'max_relative_residual': rel, 'slots': slots, 'resident_gib': resident,
+        'plain_sequential': {'value': len(base_done) / base.elapsed, 'completed': len(base_done),
+                             'unserved': base.unserved, 'p50_ms': 1e3 * base_stats['p50'],
+                             'p99_ms': 1e3 * base_stats['p99']},
+        'speedup_vs_plain_sequential': (len(done) / elapsed) / max(len(base_done) / base.elapsed, 1e-30),
     }
     print(json.dumps(out), flush=True)
 

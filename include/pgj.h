@@ -217,7 +217,9 @@ int pgj_to_f32(int64_t n, const double* x, float* y, void* stream);
 size_t pgj_inner_ws_bytes(const pgj_grid* g, int L);
 /* Per load of [l0, l0+nl): s_l = |r_l| from the FP64 state of the last
  * certification, or 0 if that load is certified; r32 = r64 / s_l; u32 = 0;
- * inner target |r32| <= rtol. state32 is a resident state block of L loads. */
+ * inner target |r32| <= max(rtol, min(target_l / (2 s_l), 1/2)), so the last
+ * step stops once the FP64 target can be met. state32 is a resident state
+ * block of L loads. */
 int pgj_inner_begin(const pgj_grid* g, int L, int l0, int nl, const double* r64, const void* state64, double rtol,
                     float* r32, float* u32, void* state32, void* ws32, void* stream);
 /* pgj_resident_init / _stage / _iterate in FP32 (plan32, FP32 fields and vectors). */

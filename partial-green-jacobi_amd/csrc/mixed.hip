@@ -18,6 +18,9 @@ __global__ void convert_f32(const double* x, float* y, int64_t n) {
     y[i] = float(x[i]);
 }
 
+// The inner target is rtol, or 1/2 of the reduction the outer target still
+// needs if that is looser (the last refinement step then stops as soon as
+// the FP64 certification can pass instead of over-solving).
 __global__ void inner_scale(StateView outer, StateView inner, double* rscale, double rtol) {
   const int l = blockIdx.x;
   if (threadIdx.x != 0) return;
@@ -25,7 +28,9 @@ __global__ void inner_scale(StateView outer, StateView inner, double* rscale, do
   const bool open = isfinite(abs_r) && abs_r * abs_r > outer.target2[l];
   rscale[l] = open ? abs_r : 0.0;
   inner.norm[l] = open ? 1.0 : 0.0;
-  inner.target2[l] = rtol * rtol;
+  const double need = open ? 0.5 * sqrt(outer.target2[l]) / abs_r : 0.0;
+  const double t = fmax(rtol, fmin(need, 0.5));
+  inner.target2[l] = t * t;
   inner.it[l] = 0;
   inner.bad[l] = 0;
   inner.active[l] = 0;

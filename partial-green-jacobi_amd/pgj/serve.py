@@ -326,6 +326,78 @@ class Server:
         return done
 
 
+class SequentialServer:
+    """The plain drop-in serving the same requests one at a time (the comparison point of ``Server``).
+
+    One FP64 ``Resident`` (parallel load mode, one group of six) per grid size; each client's six warm-start
+    displacements are kept on the device and copied into the solver for its requests. No batching: a request
+    starts when the previous one has finished.
+    """
+
+    def __init__(self, sizes, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, device='cuda'):
+        from .resident import Resident
+        self.rtol, self.device = rtol, device
+        self.volume, self.move, self.rmin = volume, move, rmin
+        self.solvers = {int(n): Resident(int(n), LOADS6, 'parallel', batch=6, device=device).capture()
+                        for n in sizes}
+        self.workers = {}
+        self.warm = {}
+        self.unserved = 0
+
+    def _worker(self, n, emin):
+        w = self.workers.get((n, emin))
+        if w is None:
+            w = PaperOuter(n, self.volume, rmin=self.rmin, beta=4.0, eta=0.5, emin=emin, move=self.move,
+                           device=self.device).capture()
+            self.workers[(n, emin)] = w
+        return w
+
+    def _step(self, c):
+        res = self.solvers[c.n]
+        w = self._worker(c.n, c.emin)
+        warm = self.warm.get(c.cid)
+        res.solutions.copy_(warm) if warm is not None else res.solutions.zero_()
+        w.set_initial(c.x)
+        w.evaluate()
+        res.material(w.a)
+        res.select(c.alpha)
+        infos, _ = res.solve(rtol=self.rtol)
+        _, local, _ = res.energy('shear')
+        H = res.effective_tensor()
+        count = w.step(local)
+        c.x.copy_(w.new)
+        self.warm[c.cid] = res.solutions.clone()
+        return infos, H, count
+
+    def run(self, clients, arrivals, horizon=None, deadline=None):
+        for c in clients:
+            self._worker(c.n, c.emin)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        queue = sorted(a for a in arrivals if horizon is None or a[0] <= horizon)
+        done = []
+        for k, (at, cid) in enumerate(queue):
+            if deadline is not None and now() > deadline:
+                self.unserved = len(queue) - k
+                break
+            wait = at - now()
+            if wait > 0:
+                time.sleep(wait)
+            req = Request(clients[cid], at)
+            req.start = now()
+            infos, H, count = self._step(clients[cid])
+            torch.cuda.synchronize()
+            req.end = now()
+            clients[cid].done += 1
+            clients[cid].history.append(dict(CH=H.cpu().numpy(), iterations=[i['iterations'] for i in infos],
+                                             bisections=count, latency=req.end - req.arrival,
+                                             max_rel=max(i['relative_residual'] for i in infos)))
+            done.append(req)
+        self.elapsed = now()
+        return done
+
+
 def poisson_arrivals(rate, horizon, clients_steps, seed):
     """Arrival times of a Poisson process of ``rate``/s over ``horizon`` s, each assigned to a client.
 

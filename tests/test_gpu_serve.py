@@ -76,3 +76,13 @@ def test_client_matches_plain_path():
             assert abs(infos[j]['iterations'] - h['iterations'][j]) <= 1
         np.testing.assert_allclose(h['CH'], H, rtol=0, atol=1e-10 * np.abs(H).max())
     assert float((c[0].x - outer.x).abs().max()) < 1e-8
+
+
+def test_sequential_baseline_serves_in_order():
+    from pgj.serve import SequentialServer
+    c = clients()[:2]
+    base = SequentialServer((16, 24))
+    done = base.run(c, [(0.0, 0), (0.0, 1), (0.001, 0)])
+    assert len(done) == 3 and base.unserved == 0
+    assert [len(x.history) for x in c] == [2, 1]
+    assert all(h['max_rel'] <= 1e-8 for x in c for h in x.history)
