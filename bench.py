@@ -70,10 +70,10 @@ def stage_bytes(L, n):
     N = n ** 3
     nf = n * n * (n // 2 + 1)
     return {
-        'hex8_rowmarch': L * (2 * 24 * N + 2 * 24 * N) + 8 * N,
+        'hex8_rowmarch': L * (24 * N + 24 * N) + 8 * N,
         'fxy': L * (6 * 24 * N + 48 * nf) + 24 * N,
         'fz': L * 2 * 48 * nf + 48 * nf + 48 * nf,
-        'fxyi': L * (48 * nf + 24 * N) + 24 * N,
+        'fxyi': L * (48 * nf + 2 * 24 * N) + 24 * N,
     }
 
 
@@ -88,14 +88,14 @@ def probe_stages(res, iters):
     times = {k: [] for k in names}
     bufs = [res.p0, res.p1]
     for i in range(iters):
-        pold, pnew = bufs[i % 2], bufs[(i + 1) % 2]
+        p, p_next = bufs[i % 2], bufs[(i + 1) % 2]
         for stage, name in enumerate(names):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, 0, res.batch, _lib.ptr(res.op._plan),
                       _lib.ptr(res.op._material_field), _lib.ptr(res.scale), _lib.ptr(res.scale_hat),
-                      _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(pold), _lib.ptr(pnew), _lib.ptr(res.q),
-                      _lib.ptr(res.zv), _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())
+                      _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(p), _lib.ptr(p_next), _lib.ptr(res.q),
+                      _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())
             e1.record()
             times[name].append((e0, e1))
     torch.cuda.synchronize()

@@ -179,23 +179,26 @@ int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, int nl, cons
  * Replaces Resident.certify (resident.py:133-136). */
 int pgj_resident_certify(const pgj_grid* g, int L, int group, int l0, int nl, const double* a, double* u,
                          const double* rhs, double* r, void* state, void* ws, void* stream);
-/* active = |r|^2 > target2 & iterations < 4000 & !bad; z = M^-1 r; first
- * direction armed. Replaces Resident.initialize (resident.py:119-123). */
+/* active = |r|^2 > target2 & iterations < 4000 & !bad; first direction
+ * p = M^-1 r (written to p). Replaces Resident.initialize
+ * (resident.py:119-123). */
 int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* s,
-                      const void* s_hat, const double* r, double* z, void* state, void* ws, void* stream);
+                      const void* s_hat, const double* r, double* p, void* state, void* ws, void* stream);
 /* `iters` (even) masked PCG iterations; p0/p1 are ping-pong direction
- * buffers, q and z are scratch vectors, all (L, 3N). Replaces
+ * buffers (the first iteration reads p0, as written by pgj_resident_init),
+ * q is a scratch vector, all (L, 3N). The preconditioned residual z is not
+ * stored: the inverse FFT pass writes p_next = z + beta p directly. Replaces
  * Resident.iteration with update_ur/update_p (resident.py:39-58, 125-131). */
 int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* a,
                          const double* s, const void* s_hat, double* u, double* r, double* p0, double* p1, double* q,
-                         double* z, void* state, void* ws, int iters, void* stream);
-/* One stage of one iteration (0: direction update + K apply, 1: vector update +
- * forward xy FFT, 2: z pass + Green, 3: inverse xy FFT + projection), reading
- * the direction from pold and writing pnew; pgj_resident_iterate is these four
+                         void* state, void* ws, int iters, void* stream);
+/* One stage of one iteration (0: q = K p, 1: vector update + forward xy FFT,
+ * 2: z pass + Green, 3: inverse xy FFT + projection + p_next = z + beta p),
+ * with p the direction of this iteration; pgj_resident_iterate is these four
  * stages with p0/p1 alternating. Exposed so each kernel can be timed alone. */
 int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* a,
-                       const double* s, const void* s_hat, double* u, double* r, double* pold, double* pnew, double* q,
-                       double* z, void* state, void* ws, int stage, void* stream);
+                       const double* s, const void* s_hat, double* u, double* r, double* p, double* p_next, double* q,
+                       void* state, void* ws, int stage, void* stream);
 
 /* ------------------------------------------ mixed precision (FP32 inner) */
 /* FP32 inner PCG inside FP64 iterative refinement. The inner solver is the
@@ -224,12 +227,12 @@ int pgj_inner_begin(const pgj_grid* g, int L, int l0, int nl, const double* r64,
                     float* r32, float* u32, void* state32, void* ws32, void* stream);
 /* pgj_resident_init / _stage / _iterate in FP32 (plan32, FP32 fields and vectors). */
 int pgj_inner_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32, const float* s,
-                   const void* s_hat, const float* r, float* z, void* state, void* ws, void* stream);
+                   const void* s_hat, const float* r, float* p, void* state, void* ws, void* stream);
 int pgj_inner_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32, const float* a,
-                    const float* s, const void* s_hat, float* u, float* r, float* pold, float* pnew, float* q, float* z,
+                    const float* s, const void* s_hat, float* u, float* r, float* p, float* p_next, float* q,
                     void* state, void* ws, int stage, void* stream);
 int pgj_inner_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32, const float* a,
-                      const float* s, const void* s_hat, float* u, float* r, float* p0, float* p1, float* q, float* z,
+                      const float* s, const void* s_hat, float* u, float* r, float* p0, float* p1, float* q,
                       void* state, void* ws, int iters, void* stream);
 /* u64 += s_l * u32 for the loads of [l0, l0+nl) (FP64 accumulation). */
 int pgj_inner_end(const pgj_grid* g, int L, int l0, int nl, const float* u32, const void* ws32, double* u64,

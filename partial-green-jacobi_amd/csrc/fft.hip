@@ -344,6 +344,15 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
     for (int c = 0; c < 3; ++c)
       mean[c] = wave_reduce_partials(A.pg + int64_t(l) * cnt * 4 + 1 + c, cnt, 4) / double(A.N);
   }
+  // PCG: the output is the next direction p = z + beta p (z alone right after init)
+  double beta = 0.0;
+  const bool update = MODE == kFxyiPCG && A.p != nullptr && !A.st.first[l];
+  if (update) {
+    const int cnt = f.nkb * ny;
+    const double rz_new = wave_reduce_partials(A.pg + int64_t(l) * cnt * 4, cnt, 4);
+    const double rz = A.st.rz[l];
+    beta = rz_new / (rz != 0.0 ? rz : 1.0);
+  }
   // 1. load the half spectrum of the plane
   const double2* in = A.spec + ((int64_t(l) * 3 + c0) * A.nz + z) * int64_t(ny) * f.nkx;
   const int64_t total = int64_t(comps) * ny * f.nkx;
@@ -406,7 +415,8 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
     const double w = even ? reinterpret_cast<const double*>(row)[x] : row[x].x;
     const int64_t gnode = int64_t(z) * plane_nodes + node;
     const double v = A.s ? A.s[vindex(A, design_of(l, A.group), c, gnode)] * w - mean[c] : w;
-    A.zout[vindex(A, l, c, gnode)] = v;
+    const int64_t oi = vindex(A, l, c, gnode);
+    A.zout[oi] = update ? fma(beta, A.p[oi], v) : v;
   }
 }
 

@@ -27,12 +27,12 @@ struct FFTArgsT {
   const T* rin;          // fxy pre: input residual
   T* u;                  // fxy pcg
   T* r;                  // fxy pcg
-  const T* p;            // fxy pcg
+  const T* p;            // fxy pcg: direction for u += alpha p; fxyi pcg: direction for p_next = z + beta p
   const T* q;            // fxy pcg
   const T* s;            // scale (nullable)
   const C* shat;         // spectrum of s (nullable)
   C* spec;               // spectrum buffer (L x 3 x nf)
-  T* zout;               // fxyi output
+  T* zout;               // fxyi output: z (preconditioner), or the next PCG direction
   double* pr;            // fxy partials  [l][z][c][2] = (sum r^2, sum r)
   double* pg;            // fz partials   [l][blk][4]  = (v.w, s.w_x, s.w_y, s.w_z)
   const double* ppq;     // hex8 partials [l][blk], ppq_count blocks in rows of ppq_stride

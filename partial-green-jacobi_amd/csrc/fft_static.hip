@@ -278,8 +278,17 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
     A.st.bad[l] = A.st.bad_next[l];
   }
   if (!A.st.active_next[l]) return;
+  const double* pgl = A.pg + int64_t(l) * C::NKB * N * 4;
   double mean = 0.0;
-  if (A.shat) mean = wave_reduce_partials(A.pg + int64_t(l) * C::NKB * N * 4 + 1 + c, C::NKB * N, 4) / double(NN);
+  if (A.shat) mean = wave_reduce_partials(pgl + 1 + c, C::NKB * N, 4) / double(NN);
+  // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
+  const bool update = A.p != nullptr && !A.st.first[l];
+  double beta = 0.0;
+  if (update) {
+    const double rz_new = wave_reduce_partials(pgl, C::NKB * N, 4);
+    const double rz = A.st.rz[l];
+    beta = rz_new / (rz != 0.0 ? rz : 1.0);
+  }
   const X* in = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
   for (int e = threadIdx.x; e < N * C::NKX; e += T) plane[e] = in[e];
   __syncthreads();
@@ -310,9 +319,10 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
   const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
+  const X* __restrict__ pp = update ? reinterpret_cast<const X*>(A.p + off) : nullptr;
   X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
   const bool scaled = A.s != nullptr;
-  const R m = R(mean);
+  const R m = R(mean), tb = R(beta);
 #pragma unroll 4
   for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
     const int x = e % N, y = e / N;
@@ -321,6 +331,10 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
     if (scaled) {
       const X sv = sp[e / 2];
       v = c2(sv.x * w.x - m, sv.y * w.y - m);
+    }
+    if (update) {
+      const X po = pp[e / 2];
+      v = c2(fma(tb, po.x, v.x), fma(tb, po.y, v.y));
     }
     zp[e / 2] = v;
   }

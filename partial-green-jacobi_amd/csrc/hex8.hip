@@ -7,12 +7,10 @@
 
 namespace pgj {
 
-// Raw inputs of one node row segment: the vector itself (apply) or the pair
-// (z, p_old) the fused update p = z + beta p_old is formed from (PCG).
+// Raw inputs of one node row segment (the input vector: u, or the PCG direction p).
 template <int MODE, int R, int XPL, typename T>
 struct Rows {
   T v[R + 1][XPL][3];
-  T w[MODE == kModePCG ? R + 1 : 1][XPL][3];
 };
 
 // Row-march HEX8 apply. Lanes map to x (XPL consecutive x per lane), each of
@@ -47,16 +45,8 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   const bool lane_on = lane < c.nl;
   const unsigned xo = unsigned(xl * XPL);
 
-  double beta = 0.0;
-  bool first = false;
   if (MODE == kModePCG) {
     if (!A.st.active[l]) return;
-    first = A.st.first[l] != 0;
-    if (!first) {
-      const double rz_new = wave_reduce_partials(A.pg + int64_t(l) * A.pg_stride, A.pg_count, 4);
-      const double rz = A.st.rz[l];
-      beta = rz_new / (rz != 0.0 ? rz : 1.0);
-    }
   } else if (A.mask && !A.mask[l]) {
     // Inactive load: zeros, as the reference's pre-zeroed output.
     T* o = A.out + size_t(l) * 3 * N;
@@ -76,8 +66,6 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   const int rows = halo ? 1 : R;  // element rows computed by this wave
   // uniform base pointers per component (SoA) or per load (AoS)
   const T* __restrict__ in = A.u + size_t(l) * 3 * N;
-  const T* __restrict__ old = SOA ? A.pold + size_t(l) * 3 * N : nullptr;
-  T* __restrict__ pn = SOA ? A.pnew + size_t(l) * 3 * N : nullptr;
   T* __restrict__ out = A.out + size_t(l) * 3 * N;
   // element offsets in bytes (32-bit): uniform base + 32-bit offset selects the SGPR-base addressing form
   auto at = [&](unsigned node, int d) -> unsigned { return ES * (SOA ? unsigned(d) * N + node : 3u * node + unsigned(d)); };
@@ -98,33 +86,20 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
       for (int xi = 0; xi < XPL; ++xi) {
         const unsigned node = rb + xo + xi;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          dst.v[j][xi][d] = ld(in, at(node, d));
-          // p_old is read unconditionally (ignored right after init) so the load keeps its SGPR base
-          if constexpr (MODE == kModePCG) dst.w[j][xi][d] = ld(old, at(node, d));
-        }
+        for (int d = 0; d < 3; ++d) dst.v[j][xi][d] = ld(in, at(node, d));
         if (j < R) arow[j][xi] = ld(mat, ES * node);
       }
     }
   };
-  // Node values of a fetched plane; p_new is stored for owned rows.
-  const T tbeta = T(beta);
-  auto settle = [&](int plane, const Rows<MODE, R, XPL, T>& src, T (&dst)[R + 1][XPL][3], bool write_p) {
+  // Node values of a fetched plane.
+  auto settle = [&](const Rows<MODE, R, XPL, T>& src, T (&dst)[R + 1][XPL][3]) {
 #pragma unroll
     for (int j = 0; j <= R; ++j) {
       if (j > rows) break;
-      const unsigned rb = row_of(plane, j);
 #pragma unroll
       for (int xi = 0; xi < XPL; ++xi)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          T v = src.v[j][xi][d];
-          if constexpr (MODE == kModePCG) {
-            if (!first) v = fma(tbeta, src.w[j][xi][d], v);
-            if (write_p && j < rows && !halo && lane_on) st(pn, at(rb + xo + xi, d), v);
-          }
-          dst[j][xi][d] = v;
-        }
+        for (int d = 0; d < 3; ++d) dst[j][xi][d] = src.v[j][xi][d];
     }
   };
 
@@ -159,7 +134,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
     };
     Rows<MODE, R, XPL, T> raw;
     fetch(wrapi(z0 - 1, nz), raw, aL);
-    settle(wrapi(z0 - 1, nz), raw, nodeU, false);
+    settle(raw, nodeU);
     plane_modes(nodeU, PL);
 #pragma unroll
     for (int j = 0; j < R; ++j)
@@ -174,7 +149,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
     fetch(z0, raw, aU);
     for (int li = 0; li <= c.ZC; ++li) {
       const int zl = wrapi(z0 - 1 + li, nz), zu = wrapi(z0 + li, nz);
-      settle(zu, raw, nodeU, li < c.ZC);
+      settle(raw, nodeU);
       if (li < c.ZC) fetch(wrapi(z0 + li + 1, nz), raw, aN);  // prefetch the next plane
 #pragma unroll
       for (int j = 0; j <= R; ++j)
@@ -290,11 +265,11 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
 
   Rows<MODE, R, XPL, T> raw;
   fetch(wrapi(z0 - 1, nz), raw, aL);
-  settle(wrapi(z0 - 1, nz), raw, nodeL, false);
+  settle(raw, nodeL);
   fetch(z0, raw, aU);
   for (int li = 0; li <= c.ZC; ++li) {
     const int zl = wrapi(z0 - 1 + li, nz), zu = wrapi(z0 + li, nz);
-    settle(zu, raw, nodeU, li < c.ZC);
+    settle(raw, nodeU);
     if (li < c.ZC) fetch(wrapi(z0 + li + 1, nz), raw, aN);  // prefetch the next plane
 #pragma unroll
     for (int j = 0; j <= R; ++j)

@@ -16,15 +16,10 @@ struct RowArgsT {
   RowMarch cfg;
   ElementCoef coef;
   const T* a;
-  const T* u;            // apply: input vectors; PCG: preconditioned residual z
-  T* out;                // apply: K u; PCG: q = K p_new
+  const T* u;            // apply: input vectors; PCG: the direction p
+  T* out;                // apply: K u; PCG: q = K p
   const uint8_t* mask;   // apply: optional active mask
-  // PCG fusion
-  const T* pold;
-  T* pnew;
-  StateView st;
-  const double* pg;      // FZ partials (rz part first of 4), per load stride pg_stride
-  int pg_count, pg_stride;
+  StateView st;          // PCG: active mask
   double* ppq;           // per-block p.q partials, per load stride ppq_stride
   int ppq_stride;
   int group;             // loads per design (a is (designs, N)); 0: one shared a

@@ -1,9 +1,10 @@
 // Resident multi-load PCG: set-up, certification, initialization and the
 // fused iteration (four kernels per iteration, no host round trip):
-//   1. hex8_rowmarch<PCG>: p = z + beta p (or z right after init), q = K p, p.q partials
+//   1. hex8_rowmarch<PCG>: q = K p, p.q partials
 //   2. fxy<PCG>          : alpha = rz / p.q; u += alpha p; r -= alpha q; forward xy FFT of s*r
 //   3. fz<PCG>           : masks, latches, iteration count; z-FFT, projection, Green, Parseval partials
-//   4. fxyi<PCG>         : inverse FFT, z = P(s*w); publishes the new active mask
+//   4. fxyi<PCG>         : inverse FFT, z = P(s*w), next direction p = z + beta p (z alone after init);
+//                          publishes the new active mask. z itself is never stored.
 // Per-load scalar semantics follow exp/asg_gpu_resident_20261103/resident.py:39-58 and 119-136.
 #include <cmath>
 
@@ -181,19 +182,20 @@ int fxyi(FFTArgsT<T>& A, const pgj_grid& g, int L, hipStream_t s) {
   return PGJ_ERR_UNSUPPORTED;
 }
 
-// z = M^-1 r on a slice with the active-mask set-up (precision T); A carries the plan.
+// First direction p = M^-1 r on a slice with the active-mask set-up (precision T); A carries the plan.
 template <typename T>
 int resident_init_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgsT<T> A, const T* sc,
-                    const cplx<T>* s_hat, const T* r, T* z, void* state, void* ws, hipStream_t s) {
+                    const cplx<T>* s_hat, const T* r, T* p, void* state, void* ws, hipStream_t s) {
   Slice sl{};
   if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
-  PGJ_REQUIRE(r && z && state && ws, "invalid init arguments");
+  PGJ_REQUIRE(r && p && state && ws, "invalid init arguments");
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   ResidentBuffersT<T> b = resident_buffers<T>(*g, L, ws, l0);
   A.rin = r + sl.vec();
   A.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
   A.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
-  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = z + sl.vec();
+  A.spec = b.spec; A.pr = b.pr; A.pg = b.pg; A.zout = p + sl.vec();
+  A.p = nullptr;   // right after init the direction is the preconditioned residual itself
   A.ppq = b.ppq; A.ppq_count = hex8_blocks_per_load(*g, nl, int(sizeof(T))); A.ppq_stride = hex8_blocks_bound(*g);
   A.st = state_view(state, L, l0);
   A.soa = 1;
@@ -203,33 +205,33 @@ int resident_init_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgs
   return fxyi(A, *g, nl, s);
 }
 
-// One of the four fused iteration stages on a slice (precision T).
+// One of the four fused iteration stages on a slice (precision T). The
+// direction of this iteration is p (read by stages 0, 1, 3); stage 3 writes
+// the next direction to p_next.
 template <typename T>
 int resident_stage_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgsT<T> F, const T* a, const T* sc,
-                     const cplx<T>* s_hat, T* u, T* r, T* pold, T* pnew, T* q, T* z, void* state, void* ws,
-                     int stage, hipStream_t s) {
+                     const cplx<T>* s_hat, T* u, T* r, T* p, T* p_next, T* q, void* state, void* ws, int stage,
+                     hipStream_t s) {
   Slice sl{};
   if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
-  PGJ_REQUIRE(a && u && r && pold && pnew && q && z && state && ws, "invalid stage arguments");
+  PGJ_REQUIRE(a && u && r && p && p_next && q && state && ws, "invalid stage arguments");
   PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   PGJ_REQUIRE(rowmarch_config(*g, nl).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
   ResidentBuffersT<T> b = resident_buffers<T>(*g, L, ws, l0);
   StateView st = state_view(state, L, l0);
   const int64_t v = sl.vec();
-  F.u = u + v; F.r = r + v; F.q = q + v; F.p = pnew + v;
+  F.u = u + v; F.r = r + v; F.q = q + v; F.p = p + v;
   F.s = sc ? sc + sl.design() * 3 * sl.N : nullptr;
   F.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
-  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = z + v;
+  F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = p_next + v;
   F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl, int(sizeof(T))); F.ppq_stride = hex8_blocks_bound(*g);
   F.st = st;
   F.soa = 1;
   F.group = group;
   if (stage == 0) {
-    const int pg_count = F.f.nkb * g->ny;
     RowArgsT<T> K{};
-    K.a = a + sl.design() * sl.N; K.u = z + v; K.out = q + v; K.pold = pold + v; K.pnew = pnew + v; K.st = st;
-    K.pg = b.pg; K.pg_count = pg_count; K.pg_stride = pg_count * 4;
+    K.a = a + sl.design() * sl.N; K.u = p + v; K.out = q + v; K.st = st;
     K.ppq = b.ppq; K.ppq_stride = F.ppq_stride;
     K.group = group;
     return launch_rowmarch(*g, nl, kModePCG, K, s);
@@ -255,34 +257,34 @@ extern "C" int pgj_resident_scale(const pgj_grid* g, const void* plan, const dou
 }
 
 extern "C" int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
-                                 const double* sc, const void* s_hat, const double* r, double* z, void* state,
+                                 const double* sc, const void* s_hat, const double* r, double* p, void* state,
                                  void* ws, void* stream) {
   if (int e = check_fft(g)) return e;
   PGJ_REQUIRE(plan, "invalid init arguments");
   return resident_init_t<double>(g, L, group, l0, nl, fft_args(*g, plan), sc, static_cast<const double2*>(s_hat), r,
-                                 z, state, ws, as_stream(stream));
+                                 p, state, ws, as_stream(stream));
 }
 
 extern "C" int pgj_resident_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
                                   const double* a, const double* sc, const void* s_hat, double* u, double* r,
-                                  double* pold, double* pnew, double* q, double* z, void* state, void* ws, int stage,
+                                  double* p, double* p_next, double* q, void* state, void* ws, int stage,
                                   void* stream) {
   if (int e = check_fft(g)) return e;
   PGJ_REQUIRE(plan, "invalid stage arguments");
   return resident_stage_t<double>(g, L, group, l0, nl, fft_args(*g, plan), a, sc, static_cast<const double2*>(s_hat),
-                                  u, r, pold, pnew, q, z, state, ws, stage, as_stream(stream));
+                                  u, r, p, p_next, q, state, ws, stage, as_stream(stream));
 }
 
 extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
                                     const double* a, const double* sc, const void* s_hat, double* u, double* r,
-                                    double* p0, double* p1, double* q, double* z, void* state, void* ws, int iters,
+                                    double* p0, double* p1, double* q, void* state, void* ws, int iters,
                                     void* stream) {
   PGJ_REQUIRE(iters >= 2 && iters % 2 == 0, "iteration count must be even and >= 2, got %d", iters);
   for (int i = 0; i < iters; ++i) {
-    double* pold = (i % 2 == 0) ? p0 : p1;
-    double* pnew = (i % 2 == 0) ? p1 : p0;
+    double* p = (i % 2 == 0) ? p0 : p1;
+    double* p_next = (i % 2 == 0) ? p1 : p0;
     for (int stage = 0; stage < 4; ++stage)
-      if (int e = pgj_resident_stage(g, L, group, l0, nl, plan, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws,
+      if (int e = pgj_resident_stage(g, L, group, l0, nl, plan, a, sc, s_hat, u, r, p, p_next, q, state, ws,
                                      stage, stream))
         return e;
   }
@@ -295,33 +297,33 @@ FFTArgsT<float> fft_args32(const pgj_grid& g, const void* plan32);
 }
 
 extern "C" int pgj_inner_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32,
-                              const float* sc, const void* s_hat, const float* r, float* z, void* state, void* ws,
+                              const float* sc, const void* s_hat, const float* r, float* p, void* state, void* ws,
                               void* stream) {
   PGJ_REQUIRE(g && plan32, "invalid inner init arguments");
   PGJ_REQUIRE(static_cube(*g), "the FP32 inner solver needs a cubic n in 16, 24, 32, 48, 64, 96, 128");
   return resident_init_t<float>(g, L, group, l0, nl, fft_args32(*g, plan32), sc, static_cast<const float2*>(s_hat), r,
-                                z, state, ws, as_stream(stream));
+                                p, state, ws, as_stream(stream));
 }
 
 extern "C" int pgj_inner_stage(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32,
-                               const float* a, const float* sc, const void* s_hat, float* u, float* r, float* pold,
-                               float* pnew, float* q, float* z, void* state, void* ws, int stage, void* stream) {
+                               const float* a, const float* sc, const void* s_hat, float* u, float* r, float* p,
+                               float* p_next, float* q, void* state, void* ws, int stage, void* stream) {
   PGJ_REQUIRE(g && plan32, "invalid inner stage arguments");
   PGJ_REQUIRE(static_cube(*g), "the FP32 inner solver needs a cubic n in 16, 24, 32, 48, 64, 96, 128");
   return resident_stage_t<float>(g, L, group, l0, nl, fft_args32(*g, plan32), a, sc, static_cast<const float2*>(s_hat),
-                                 u, r, pold, pnew, q, z, state, ws, stage, as_stream(stream));
+                                 u, r, p, p_next, q, state, ws, stage, as_stream(stream));
 }
 
 extern "C" int pgj_inner_iterate(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan32,
                                  const float* a, const float* sc, const void* s_hat, float* u, float* r, float* p0,
-                                 float* p1, float* q, float* z, void* state, void* ws, int iters, void* stream) {
+                                 float* p1, float* q, void* state, void* ws, int iters, void* stream) {
   PGJ_REQUIRE(iters >= 2 && iters % 2 == 0, "iteration count must be even and >= 2, got %d", iters);
   for (int i = 0; i < iters; ++i) {
-    float* pold = (i % 2 == 0) ? p0 : p1;
-    float* pnew = (i % 2 == 0) ? p1 : p0;
+    float* p = (i % 2 == 0) ? p0 : p1;
+    float* p_next = (i % 2 == 0) ? p1 : p0;
     for (int stage = 0; stage < 4; ++stage)
-      if (int e = pgj_inner_stage(g, L, group, l0, nl, plan32, a, sc, s_hat, u, r, pold, pnew, q, z, state, ws,
-                                  stage, stream))
+      if (int e = pgj_inner_stage(g, L, group, l0, nl, plan32, a, sc, s_hat, u, r, p, p_next, q, state, ws, stage,
+                                  stream))
         return e;
   }
   return PGJ_OK;

@@ -47,7 +47,7 @@ class DesignGroup:
         self.scale_hat = torch.zeros((self.D, 3 * nf), dtype=torch.complex128, device=dev)
         self.alpha = np.zeros(self.D)
         # SoA (L, 3, N) solver vectors; u persists between solves as the warm start
-        self.u, self.r, self.p0, self.p1, self.q, self.zv, self.rhs = (z(L, 3 * N) for _ in range(7))
+        self.u, self.r, self.p0, self.p1, self.q, self.rhs = (z(L, 3 * N) for _ in range(6))
         lib = _lib.lib()
         self.state = torch.zeros(lib.pgj_resident_state_bytes(L), dtype=torch.uint8, device=dev)
         offs = (_lib.c_size_t * 7)()
@@ -96,20 +96,20 @@ class DesignGroup:
     def initialize(self, d0=0, nd=None):
         l0, nl = self._slice(d0, nd)
         _lib.call('pgj_resident_init', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.op._plan),
-                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv),
+                  _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.p0),
                   _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
 
     def iteration(self, count=None, d0=0, nd=None):
         l0, nl = self._slice(d0, nd)
         _lib.call('pgj_resident_iterate', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.op._plan),
                   _lib.ptr(self.a), _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
-                  _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.state),
                   _lib.ptr(self.ws), int(count or self.chunk), _lib.stream())
 
-    def stage(self, k, pold, pnew):
+    def stage(self, k, p, p_next):
         _lib.call('pgj_resident_stage', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.op._plan),
                   _lib.ptr(self.a), _lib.ptr(self.scale), _lib.ptr(self.scale_hat), _lib.ptr(self.u), _lib.ptr(self.r),
-                  _lib.ptr(pold), _lib.ptr(pnew), _lib.ptr(self.q), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(p), _lib.ptr(p_next), _lib.ptr(self.q), _lib.ptr(self.state),
                   _lib.ptr(self.ws), int(k), _lib.stream())
 
     # -- results ----------------------------------------------------------------------

@@ -57,7 +57,7 @@ class MixedResident:
         self.a32 = torch.ones((self.D, N), dtype=torch.float32, device=dev)
         self.scale32 = torch.ones((self.D, 3 * N), dtype=torch.float32, device=dev)
         self.scale_hat32 = torch.zeros((self.D, 3 * nf), dtype=torch.complex64, device=dev)
-        self.d, self.r32, self.p0, self.p1, self.q, self.z = (f32(L, 3 * N) for _ in range(6))
+        self.d, self.r32, self.p0, self.p1, self.q = (f32(L, 3 * N) for _ in range(5))
         self.state32 = torch.zeros(lib.pgj_resident_state_bytes(L), dtype=torch.uint8, device=dev)
         self.ws32 = torch.empty(lib.pgj_inner_ws_bytes(g, L), dtype=torch.uint8, device=dev)
         offs = (_lib.c_size_t * 7)()
@@ -113,14 +113,14 @@ class MixedResident:
                   self.inner_rtol, _lib.ptr(self.r32), _lib.ptr(self.d), _lib.ptr(self.state32), _lib.ptr(self.ws32),
                   _lib.stream())
         _lib.call('pgj_inner_init', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.plan32),
-                  _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.r32), _lib.ptr(self.z),
+                  _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.r32), _lib.ptr(self.p0),
                   _lib.ptr(self.state32), _lib.ptr(self.ws32), _lib.stream())
 
     def iteration(self):
         """``chunk`` FP32 inner iterations of every load of the set (stopped loads exit at once)."""
         _lib.call('pgj_inner_iterate', self.op.grid, self.L, self.G, 0, self.L, _lib.ptr(self.plan32),
                   _lib.ptr(self.a32), _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.d),
-                  _lib.ptr(self.r32), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.z),
+                  _lib.ptr(self.r32), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
                   _lib.ptr(self.state32), _lib.ptr(self.ws32), self.chunk, _lib.stream())
 
     def end(self, d0=0, nd=None):

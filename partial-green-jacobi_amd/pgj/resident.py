@@ -50,7 +50,7 @@ class Resident:
         dev = torch.device(device)
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
         # solver vectors are SoA (B, 3, N) on the device; solutions are AoS (L, 3N)
-        self.u, self.r, self.p0, self.p1, self.q, self.zv, self.rhs = (z(B, 3 * N) for _ in range(7))
+        self.u, self.r, self.p0, self.p1, self.q, self.rhs = (z(B, 3 * N) for _ in range(6))
         self.solutions = z(nl, 3 * N)
         lib = _lib.lib()
         self.state = torch.zeros(lib.pgj_resident_state_bytes(B), dtype=torch.uint8, device=dev)
@@ -119,10 +119,10 @@ class Resident:
                   _lib.stream())
 
     def initialize(self):
-        """active mask and first direction z = M^-1 r (``resident.py:119-123``)."""
+        """active mask and first direction p0 = M^-1 r (``resident.py:119-123``)."""
         _lib.call('pgj_resident_init', self.op.grid, self.batch, self.batch, 0, self.batch, _lib.ptr(self.op._plan),
                   _lib.ptr(self.scale),
-                  _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.zv), _lib.ptr(self.state),
+                  _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.state),
                   _lib.ptr(self.ws), _lib.stream())
 
     def iteration(self, count=None):
@@ -130,7 +130,7 @@ class Resident:
         _lib.call('pgj_resident_iterate', self.op.grid, self.batch, self.batch, 0, self.batch, _lib.ptr(self.op._plan),
                   _lib.ptr(self.op._material_field), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
                   _lib.ptr(self.u), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
-                  _lib.ptr(self.zv), _lib.ptr(self.state), _lib.ptr(self.ws), int(count or self.chunk),
+                  _lib.ptr(self.state), _lib.ptr(self.ws), int(count or self.chunk),
                   _lib.stream())
 
     def capture(self):

@@ -42,18 +42,18 @@ LOADS6 = np.eye(6)
 def slot_bytes(n, loads=6, precision='fp64'):
     """HBM bytes one design slot of grid n holds in a resident set (vectors, spectra, fields, partials).
 
-    ``precision='mixed'``: three FP64 outer vectors and six FP32 inner vectors, FP32 fields and spectrum copies.
+    ``precision='mixed'``: three FP64 outer vectors and five FP32 inner vectors, FP32 fields and spectrum copies.
     """
     N = n ** 3
     nf = n * n * (n // 2 + 1)
     partials = loads * (n * n + n * 6 + n * n * 4 + (N + 255) // 256 * 4) * 8
     fields = N * 8 + 3 * N * 8 + 3 * nf * 16
     if precision == 'mixed':
-        vectors = 3 * loads * 3 * N * 8 + 6 * loads * 3 * N * 4
+        vectors = 3 * loads * 3 * N * 8 + 5 * loads * 3 * N * 4
         spectrum = loads * 3 * nf * (16 + 8)
         fields += N * 4 + 3 * N * 4 + 3 * nf * 8
         return vectors + spectrum + fields + 2 * partials
-    vectors = 7 * loads * 3 * N * 8
+    vectors = 6 * loads * 3 * N * 8
     spectrum = loads * 3 * nf * 16
     return vectors + spectrum + fields + partials
 

@@ -45,4 +45,4 @@ def test_host_side_queries_without_gpu():
         _lib.check(lib.pgj_hex8_apply(bad, 1, None, None, None, None, None))
     with pytest.raises(ValueError):
         _lib.check(lib.pgj_resident_iterate(g, 6, 6, 0, 6, None, None, None, None, None, None, None, None, None,
-                                            None, None, None, 3, None))
+                                            None, None, 3, None))
