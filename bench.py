@@ -71,9 +71,9 @@ def stage_bytes(L, n):
     nf = n * n * (n // 2 + 1)
     return {
         'hex8_rowmarch': L * (24 * N + 24 * N) + 8 * N,
-        'fxy': L * (6 * 24 * N + 48 * nf) + 24 * N,
+        'fxy': L * (3 * 24 * N + 48 * nf) + 24 * N,
         'fz': L * 2 * 48 * nf + 48 * nf + 48 * nf,
-        'fxyi': L * (48 * nf + 2 * 24 * N) + 24 * N,
+        'fxyi': L * (48 * nf + 4 * 24 * N) + 24 * N,
     }
 
 

@@ -3,11 +3,12 @@
 // projection and PCG vector updates fused into the xy passes.
 //
 // Data path per load (spectrum layout S[l][c][kz][ky][kx], kx < nx/2+1):
-//   fxy  : per z-plane, v = s*r (PCG: u += alpha p, r -= alpha q first);
+//   fxy  : per z-plane, v = s*r (PCG: r -= alpha q first);
 //          x-R2C (two reals per complex line + split), y-FFT; write S.
 //   fz   : per (ky, kx-chunk), z-FFT, v^ -= mean(r) s^ (projection P),
 //          w^ = G v^, Parseval partials of v.w and s.w, inverse z-FFT.
-//   fxyi : per z-plane, inverse y-FFT, x-C2R, z = s*w - mean(s*w).
+//   fxyi : per z-plane, inverse y-FFT, x-C2R, z = s*w - mean(s*w)
+//          (PCG: written as the next direction z + beta p; u += alpha p).
 #include <cmath>
 
 #include "fft_kernels.h"
@@ -150,9 +151,7 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
       sm[c] += rv;
       val = A.s ? A.s[sidx] * rv : rv;
     } else {
-      const double un = fma(alpha, A.p[idx], A.u[idx]);
       const double rn = fma(-alpha, A.q[idx], A.r[idx]);
-      A.u[idx] = un;
       A.r[idx] = rn;
       sq[c] = fma(rn, rn, sq[c]);
       sm[c] += rn;
@@ -232,6 +231,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
         A.st.it_next[l] = A.st.it[l];
         A.st.bad_next[l] = A.st.bad[l];
         A.st.first[l] = 1;
+        A.st.stopped[l] = 0;
       }
     } else {
       const bool active = A.st.active[l] != 0;
@@ -245,6 +245,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
         A.st.active_next[l] = act;
         A.st.it_next[l] = it;
         A.st.bad_next[l] = A.st.bad[l] || (active && !valid) || (active && !finite);
+        A.st.stopped[l] = active && !act;
       }
     }
     if (!act) return;
@@ -330,13 +331,28 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
   const int l = blockIdx.z;
   const int nx = A.nx, ny = A.ny;
   const int64_t plane_nodes = int64_t(nx) * ny;
+  const int64_t count = comps == 3 ? 3 * plane_nodes : plane_nodes;
+  // PCG: the displacement update u += alpha p of this iteration (alpha from the forward pass)
+  const bool move = MODE == kFxyiPCG && A.p != nullptr && A.u != nullptr;
+  const double alpha = move ? A.st.alpha[l] : 0.0;
   if (MODE == kFxyiPCG) {
     if (threadIdx.x == 0 && z == 0 && blockIdx.y == 0) {
       A.st.active[l] = A.st.active_next[l];
       A.st.it[l] = A.st.it_next[l];
       A.st.bad[l] = A.st.bad_next[l];
     }
-    if (!A.st.active_next[l]) return;
+    if (!A.st.active_next[l]) {
+      // a load stopped by this iteration's test still takes its last displacement update
+      if (move && A.st.stopped[l]) {
+        for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
+          const int64_t node = comps == 3 ? e / 3 : e;
+          const int c = comps == 3 ? int(e % 3) : c0;
+          const int64_t oi = vindex(A, l, c, int64_t(z) * plane_nodes + node);
+          A.u[oi] = fma(alpha, A.p[oi], A.u[oi]);
+        }
+      }
+      return;
+    }
   }
   double mean[3] = {0.0, 0.0, 0.0};
   if (A.shat) {
@@ -405,8 +421,7 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
   __syncthreads();
   // 4. inverse x transform
   lds_fft<true>(plane, rows_map(rows, f.rs), f.fx, A.twx, nx / f.M);
-  // 5. unpack, scale and project
-  const int64_t count = comps == 3 ? 3 * plane_nodes : plane_nodes;
+  // 5. unpack, scale and project; PCG: next direction and displacement update
   for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
     const int64_t node = comps == 3 ? e / 3 : e;
     const int c = comps == 3 ? int(e % 3) : c0;
@@ -417,6 +432,7 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
     const double v = A.s ? A.s[vindex(A, design_of(l, A.group), c, gnode)] * w - mean[c] : w;
     const int64_t oi = vindex(A, l, c, gnode);
     A.zout[oi] = update ? fma(beta, A.p[oi], v) : v;
+    if (move) A.u[oi] = fma(alpha, A.p[oi], A.u[oi]);
   }
 }
 

@@ -27,7 +27,7 @@ struct FFTArgsT {
   const T* rin;          // fxy pre: input residual
   T* u;                  // fxy pcg
   T* r;                  // fxy pcg
-  const T* p;            // fxy pcg: direction for u += alpha p; fxyi pcg: direction for p_next = z + beta p
+  const T* p;            // fxyi pcg: direction of this iteration (u += alpha p, p_next = z + beta p)
   const T* q;            // fxy pcg
   const T* s;            // scale (nullable)
   const C* shat;         // spectrum of s (nullable)

@@ -62,9 +62,7 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
   // distinct arrays: the loads of later steps may be issued ahead of the stores of earlier ones
   const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
   const X* __restrict__ rinp = reinterpret_cast<const X*>(A.rin + off);
-  X* __restrict__ up = reinterpret_cast<X*>(A.u + off);
   X* __restrict__ rp = reinterpret_cast<X*>(A.r + off);
-  const X* __restrict__ pp_ = reinterpret_cast<const X*>(A.p + off);
   const X* __restrict__ qp = reinterpret_cast<const X*>(A.q + off);
   const bool scaled = A.s != nullptr;
 #pragma unroll 4
@@ -84,13 +82,9 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
         v = rv;
       }
     } else {
-      const X uu = up[h];
       const X rr = rp[h];
-      const X pp = pp_[h];
       const X qq = qp[h];
-      const X un = c2(fma(ra, pp.x, uu.x), fma(ra, pp.y, uu.y));
       const X rn = c2(fma(-ra, qq.x, rr.x), fma(-ra, qq.y, rr.y));
-      up[h] = un;
       rp[h] = rn;
       sq = fma(double(rn.x), double(rn.x), fma(double(rn.y), double(rn.y), sq));
       sm += double(rn.x) + double(rn.y);
@@ -166,6 +160,7 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
         A.st.it_next[l] = A.st.it[l];
         A.st.bad_next[l] = A.st.bad[l];
         A.st.first[l] = 1;
+        A.st.stopped[l] = 0;
       }
     } else {
       const bool active = A.st.active[l] != 0;
@@ -179,6 +174,7 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
         A.st.active_next[l] = act;
         A.st.it_next[l] = it;
         A.st.bad_next[l] = A.st.bad[l] || (active && !valid) || (active && !finite);
+        A.st.stopped[l] = active && !act;
       }
     }
     if (!act) return;
@@ -277,7 +273,22 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
     A.st.it[l] = A.st.it_next[l];
     A.st.bad[l] = A.st.bad_next[l];
   }
-  if (!A.st.active_next[l]) return;
+  const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
+  // PCG: the displacement update u += alpha p of this iteration (alpha from the forward pass)
+  const bool move = A.p != nullptr && A.u != nullptr;
+  const R ra = move ? R(A.st.alpha[l]) : R(0);
+  if (!A.st.active_next[l]) {
+    // a load stopped by this iteration's test still takes its last displacement update
+    if (move && A.st.stopped[l]) {
+      const X* __restrict__ pp = reinterpret_cast<const X*>(A.p + off);
+      X* __restrict__ up = reinterpret_cast<X*>(A.u + off);
+      for (int h = threadIdx.x; h < C::PLANE / 2; h += T) {
+        const X po = pp[h], uo = up[h];
+        up[h] = c2(fma(ra, po.x, uo.x), fma(ra, po.y, uo.y));
+      }
+    }
+    return;
+  }
   const double* pgl = A.pg + int64_t(l) * C::NKB * N * 4;
   double mean = 0.0;
   if (A.shat) mean = wave_reduce_partials(pgl + 1 + c, C::NKB * N, 4) / double(NN);
@@ -316,10 +327,10 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
   }
   __syncthreads();
   sfft<true, C::M, typename C::Rows, T>(plane, A.twx, 2);
-  const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
-  const X* __restrict__ pp = update ? reinterpret_cast<const X*>(A.p + off) : nullptr;
+  const X* __restrict__ pp = A.p != nullptr ? reinterpret_cast<const X*>(A.p + off) : nullptr;
+  X* __restrict__ up = move ? reinterpret_cast<X*>(A.u + off) : nullptr;
   X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
   const bool scaled = A.s != nullptr;
   const R m = R(mean), tb = R(beta);
@@ -332,9 +343,13 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
       const X sv = sp[e / 2];
       v = c2(sv.x * w.x - m, sv.y * w.y - m);
     }
-    if (update) {
+    if (update || move) {
       const X po = pp[e / 2];
-      v = c2(fma(tb, po.x, v.x), fma(tb, po.y, v.y));
+      if (update) v = c2(fma(tb, po.x, v.x), fma(tb, po.y, v.y));
+      if (move) {
+        const X uo = up[e / 2];
+        up[e / 2] = c2(fma(ra, po.x, uo.x), fma(ra, po.y, uo.y));
+      }
     }
     zp[e / 2] = v;
   }

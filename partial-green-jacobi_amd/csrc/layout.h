@@ -266,7 +266,7 @@ constexpr int kMaxResidentLoads = 4096;
 // Resident state: public blocks first (documented in pgj.h), then private.
 struct StateLayout {
   size_t rz, target2, norm, absolute, it, active, bad;
-  size_t it_next, active_next, bad_next, first, alpha;
+  size_t it_next, active_next, bad_next, first, alpha, stopped;
   size_t bytes;
 };
 
@@ -285,6 +285,7 @@ inline StateLayout state_layout(int L) {
   s.bad_next = off; off += align8(size_t(L));
   s.first = off; off += align8(size_t(L));
   s.alpha = off; off += align8(size_t(L) * 8);
+  s.stopped = off; off += align8(size_t(L));
   s.bytes = align256(off);
   return s;
 }
@@ -303,6 +304,7 @@ struct StateView {
   uint8_t* bad_next;
   uint8_t* first;
   double* alpha;
+  uint8_t* stopped;   // active at the start of this iteration and stopped by its test
 };
 
 // View of loads [l0, ...) of a state block laid out for L loads.
@@ -322,6 +324,7 @@ inline StateView state_view(void* base, int L, int l0 = 0) {
   v.bad_next = reinterpret_cast<uint8_t*>(b + s.bad_next) + l0;
   v.first = reinterpret_cast<uint8_t*>(b + s.first) + l0;
   v.alpha = reinterpret_cast<double*>(b + s.alpha) + l0;
+  v.stopped = reinterpret_cast<uint8_t*>(b + s.stopped) + l0;
   return v;
 }
 

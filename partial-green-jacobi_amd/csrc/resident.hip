@@ -1,9 +1,10 @@
 // Resident multi-load PCG: set-up, certification, initialization and the
 // fused iteration (four kernels per iteration, no host round trip):
 //   1. hex8_rowmarch<PCG>: q = K p, p.q partials
-//   2. fxy<PCG>          : alpha = rz / p.q; u += alpha p; r -= alpha q; forward xy FFT of s*r
+//   2. fxy<PCG>          : alpha = rz / p.q; r -= alpha q; forward xy FFT of s*r
 //   3. fz<PCG>           : masks, latches, iteration count; z-FFT, projection, Green, Parseval partials
-//   4. fxyi<PCG>         : inverse FFT, z = P(s*w), next direction p = z + beta p (z alone after init);
+//   4. fxyi<PCG>         : inverse FFT, z = P(s*w), next direction p = z + beta p (z alone after init),
+//                          u += alpha p (also for a load stopped by this iteration's test);
 //                          publishes the new active mask. z itself is never stored.
 // Per-load scalar semantics follow exp/asg_gpu_resident_20261103/resident.py:39-58 and 119-136.
 #include <cmath>
@@ -39,6 +40,7 @@ __global__ void setup_finish(const double* part, int nblk, double rtol, double a
     st.bad[l] = 0;
     st.active[l] = 0;
     st.first[l] = 0;
+    st.stopped[l] = 0;
   }
 }
 
