@@ -45,7 +45,11 @@ inline RowMarch rowmarch_config(const pgj_grid& g, int L, int es = 8) {
   c.xpl = (nx + 63) / 64;
   if (c.xpl > 2 || nx % c.xpl != 0 || ny < 2 || nz < 2 || L < 1) return c;
   c.nl = nx / c.xpl;
-  int want_r = 1, want_w = c.xpl == 1 ? 8 : 4, want_z = 0, want_plane = 0;
+  // Default shape (measured on the PCG pass over 32^3..128^3, 1 and 6 loads):
+  // the plane-carry march with two row waves (four for FP32 with one x per
+  // lane), except FP64 with two x per lane, which runs the node march.
+  int want_r = 1, want_w = (es == 4 && c.xpl == 1) ? 4 : 2, want_z = 0;
+  int want_plane = (es == 4 || c.xpl == 1) ? 1 : 0;
   {
     int r = 0, w = 0, z = 0, p = 0;
     if (tuned_shape(g, L, es, &r, &w, &z, &p)) {
@@ -69,8 +73,12 @@ inline RowMarch rowmarch_config(const pgj_grid& g, int L, int es = 8) {
     for (int z = want_z; z >= 1; --z)
       if (nz % z == 0) { c.ZC = z; break; }
   } else {
-    for (int z = 16; z >= 1; z /= 2)
-      if (nz % z == 0 && int64_t(c.by) * (nz / z) * L >= 192) { c.ZC = z; break; }
+    // the longest march (<= 16 planes) that still launches >= 768 blocks (3 per CU), at least 2 planes
+    for (int z = 16; z >= 1; z /= 2) {
+      if (nz % z != 0) continue;
+      c.ZC = z;
+      if (int64_t(c.by) * (nz / z) * L >= 768 || z <= 2) break;
+    }
   }
   c.bz = nz / c.ZC;
   c.valid = 1;

@@ -100,17 +100,19 @@ def test_operator_properties_at_64():
 
 @pytest.mark.parametrize('grid,voxel', GRIDS, ids=[str(g) for g, _ in GRIDS])
 @pytest.mark.parametrize('zc,w', [(1, 2), (2, 4), (4, 1)])
-def test_plane_carry_march_matches_oracle(grid, voxel, zc, w):
-    """The plane-carry row-march variant (shared in-plane butterflies) against the oracle, several shapes."""
+@pytest.mark.parametrize('plane', [0, 1], ids=['node', 'plane'])
+def test_row_march_variants_match_oracle(grid, voxel, zc, w, plane):
+    """Both row-march variants (node carry; plane carry with shared in-plane butterflies) against the oracle,
+    several shapes."""
     rng, a, op, vox = make(grid, voxel, 99)
     u = rng.normal(size=(3, 3 * a.size))
     ref = hex8.apply(a, u, C0, vox)
     lib = _lib.lib()
-    _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 1, w, min(zc, grid[0]), 1))
+    _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 1, w, min(zc, grid[0]), plane))
     try:
         shape = (_lib.c_int * 5)()
         _lib.check(lib.pgj_hex8_get_shape(op.grid, 3, 8, shape))
-        assert shape[3] == 1
+        assert shape[3] == plane
         ut = torch.tensor(u, device='cuda')
         got = op.apply_batched(ut).cpu().numpy()
         assert rel(got, ref) < 1e-14

@@ -126,13 +126,14 @@ def test_resident_large_certifies(n):
 
 
 @pytest.mark.parametrize('n', [16, 32])
-def test_plane_carry_march_in_pcg(n):
-    """The resident iteration with the plane-carry HEX8 variant matches the oracle recurrence."""
+@pytest.mark.parametrize('plane', [0, 1], ids=['node', 'plane'])
+def test_row_march_variants_in_pcg(n, plane):
+    """The resident iteration with either HEX8 row-march variant matches the oracle recurrence."""
     from pgj import _lib
     a = density(n, 1e-4, 2)
     r = pgj.Resident(n, load_mode='parallel', batch=6)
     lib = _lib.lib()
-    _lib.check(lib.pgj_hex8_set_shape(r.op.grid, 6, 8, 1, 4, 4, 1))
+    _lib.check(lib.pgj_hex8_set_shape(r.op.grid, 6, 8, 1, 4, 4, plane))
     try:
         r.capture()
         r.material(torch.tensor(a, device='cuda'))
