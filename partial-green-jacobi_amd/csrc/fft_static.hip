@@ -25,14 +25,58 @@ struct Cube {
   // LDS bytes for complex values of ES bytes, plus a reduction tail of 4 values x 8 waves
   static constexpr size_t lds_xy(size_t es) { return size_t(N) * RS * es + 4 * 8 * 8; }
   static constexpr size_t lds_z(size_t es) { return size_t(3) * N * KC * es + 4 * 8 * 8; }
+  // Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
+  // capped at 4: the register budget the stream batching is compiled for.
+  static constexpr int occupancy(size_t lds, int threads) {
+    const int w = int(size_t(160 * 1024) / lds) * (threads / 64) / 4;
+    return w < 1 ? 1 : (w > 4 ? 4 : w);
+  }
+  template <typename R>
+  static constexpr int occ_xy() { return occupancy(lds_xy(2 * sizeof(R)), TXY); }
+  template <typename R>
+  static constexpr int occ_z() { return occupancy(lds_z(2 * sizeof(R)), TZ); }
   using Rows = Lines<N, N, 0, RS, 1, false>;          // x transforms (length M) on the N rows
   using Cols = Lines<NKX, NKX, 0, 1, RS, true>;       // y transforms (length N) on the NKX columns
   using ZCols = Lines<3 * KC, KC, N * KC, 1, KC, true>;  // z transforms on (component, column)
+  // vector streams of one z-plane: pairs h = threadIdx.x + i TXY (i < ITER) of
+  // consecutive x, in chunks of CH pairs per thread whose loads are issued together
+  static constexpr int HALF = PLANE / 2;
+  static constexpr int ITER = (HALF + TXY - 1) / TXY;
+  static constexpr int CH = ITER < 8 ? ITER : 8;
+  static constexpr bool EXACT = HALF % TXY == 0;
 };
+
+// Global -> LDS copy of COUNT values: every load of a thread is issued before
+// its first LDS store (surplus lanes load a clamped, in-bounds index and store
+// nothing). map(e) is the global index of LDS slot e.
+template <int COUNT, int T, typename X, typename Map>
+__device__ inline void stage_in(X* lds, const X* __restrict__ src, Map map) {
+  constexpr int PER = (COUNT + T - 1) / T;
+  X v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = COUNT % T == 0 ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, COUNT - 1);
+    v[k] = src[map(e)];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = int(threadIdx.x) + k * T;
+    if (COUNT % T == 0 || e < COUNT) lds[e] = v[k];
+  }
+}
+
+// Pair index of chunk slot i of a vector stream, clamped in bounds for loads.
+template <int N>
+__device__ inline int pair_of(int i) {
+  using C = Cube<N>;
+  const int h = int(threadIdx.x) + i * C::TXY;
+  return C::EXACT ? h : min(h, C::HALF - 1);
+}
 
 // ------------------------------------------------------------ forward xy
 template <int N, int MODE, typename R>
-__global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
+__global__ __launch_bounds__(Cube<N>::TXY) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_xy<R>())))
+void fxy_static(FFTArgsT<R> A) {
   using C = Cube<N>;
   using X = cplx<R>;
   constexpr int T = C::TXY;
@@ -41,62 +85,70 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
   double* red = reinterpret_cast<double*>(plane + N * C::RS);
   const int z = blockIdx.x, c = blockIdx.y, l = blockIdx.z;
   const int64_t NN = int64_t(N) * N * N;
-  double alpha = 0.0;
-  if (MODE == kFxyPCG) {
-    if (!A.st.active[l]) return;
-    const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
-    const double rz = wave_reduce_partials(A.pg + int64_t(l) * C::NKB * N * 4, C::NKB * N, 4);
-    const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
-    alpha = valid ? rz / pq : 0.0;
-    if (threadIdx.x == 0 && z == 0 && c == 0) {
-      A.st.rz[l] = rz;
-      A.st.alpha[l] = alpha;
-      A.st.first[l] = 0;
-    }
-  }
+  if (MODE == kFxyPCG && !A.st.active[l]) return;
   const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   double sq = 0.0, sm = 0.0;
-  // two nodes (x even, x+1) per step: one packed complex of the x transform
-  const R ra = R(alpha);
-  // distinct arrays: the loads of later steps may be issued ahead of the stores of earlier ones
+  // two nodes (x even, x+1) per pair: one packed complex of the x transform
   const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
   const X* __restrict__ rinp = reinterpret_cast<const X*>(A.rin + off);
   X* __restrict__ rp = reinterpret_cast<X*>(A.r + off);
   const X* __restrict__ qp = reinterpret_cast<const X*>(A.q + off);
   const bool scaled = A.s != nullptr;
-#pragma unroll 4
-  for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
-    const int h = e / 2;
-    X v;
-    if (MODE == kFxyScale) {
-      v = sp[h];
-    } else if (MODE == kFxyPre) {
-      const X rv = rinp[h];
-      sq = fma(double(rv.x), double(rv.x), fma(double(rv.y), double(rv.y), sq));
-      sm += double(rv.x) + double(rv.y);
-      if (scaled) {
-        const X sv = sp[h];
-        v = c2(sv.x * rv.x, sv.y * rv.y);
+  double alpha = 0.0;
+#pragma unroll
+  for (int i0 = 0; i0 < C::ITER; i0 += C::CH) {
+    // every load of the chunk is issued before the first use
+    X va[C::CH], vb[C::CH], vs[C::CH];
+#pragma unroll
+    for (int i = 0; i < C::CH; ++i) {
+      const int h = pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1);
+      if (MODE == kFxyScale) {
+        va[i] = sp[h];
+      } else if (MODE == kFxyPre) {
+        va[i] = rinp[h];
       } else {
-        v = rv;
+        va[i] = rp[h];
+        vb[i] = qp[h];
       }
-    } else {
-      const X rr = rp[h];
-      const X qq = qp[h];
-      const X rn = c2(fma(-ra, qq.x, rr.x), fma(-ra, qq.y, rr.y));
-      rp[h] = rn;
-      sq = fma(double(rn.x), double(rn.x), fma(double(rn.y), double(rn.y), sq));
-      sm += double(rn.x) + double(rn.y);
-      if (scaled) {
-        const X sv = sp[h];
-        v = c2(sv.x * rn.x, sv.y * rn.y);
-      } else {
-        v = rn;
+      vs[i] = c2(R(1), R(1));
+    }
+    // PCG: the step length is reduced while the r and q loads are in flight
+    if (MODE == kFxyPCG && i0 == 0) {
+      const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
+      const double rz = wave_reduce_partials(A.pg + int64_t(l) * C::NKB * N * 4, C::NKB * N, 4);
+      const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
+      alpha = valid ? rz / pq : 0.0;
+      if (threadIdx.x == 0 && z == 0 && c == 0) {
+        A.st.rz[l] = rz;
+        A.st.alpha[l] = alpha;
+        A.st.first[l] = 0;
       }
     }
-    const int x = e % N, y = e / N;
-    plane[y * C::RS + x / 2] = v;
+    if (MODE != kFxyScale && scaled) {
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+    }
+    const R ra = R(alpha);
+#pragma unroll
+    for (int i = 0; i < C::CH; ++i) {
+      const int h = int(threadIdx.x) + (i0 + i) * T;
+      if (i0 + i >= C::ITER || (!C::EXACT && h >= C::HALF)) continue;
+      X v = va[i];
+      if (MODE != kFxyScale) {
+        X rn = va[i];
+        if (MODE == kFxyPCG) {
+          rn = c2(fma(-ra, vb[i].x, va[i].x), fma(-ra, vb[i].y, va[i].y));
+          rp[h] = rn;
+        }
+        sq = fma(double(rn.x), double(rn.x), fma(double(rn.y), double(rn.y), sq));
+        sm += double(rn.x) + double(rn.y);
+        v = c2(vs[i].x * rn.x, vs[i].y * rn.y);
+      }
+      const int e = 2 * h;
+      const int x = e % N, y = e / N;
+      plane[y * C::RS + x / 2] = v;
+    }
   }
   if (MODE != kFxyScale) {
     double t[2] = {sq, sm};
@@ -129,7 +181,8 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxy_static(FFTArgsT<R> A) {
 
 // ---------------------------------------------------------------- z + Green
 template <int N, int MODE, typename R>
-__global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
+__global__ __launch_bounds__(Cube<N>::TZ) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_z<R>())))
+void fz_static(FFTArgsT<R> A) {
   using C = Cube<N>;
   using X = cplx<R>;
   constexpr int T = C::TZ, KC = C::KC, NKX = C::NKX;
@@ -185,10 +238,8 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
       mean[c] = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
   }
   X* S = A.spec + int64_t(l) * 3 * C::NF;
-  for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
-    const int kcol = e % KC, cz = e / KC;  // cz = c * N + zz
-    cols[e] = S[(int64_t(cz) * N + y) * NKX + kx0 + kcol];
-  }
+  // LDS slot e = cz * KC + kcol with cz = c * N + zz
+  stage_in<3 * N * KC, T>(cols, S, [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; });
   __syncthreads();
   sfft<false, N, typename C::ZCols, T>(cols, A.twz, 1);
   if (MODE == kFzScale) {
@@ -199,31 +250,49 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
     return;
   }
   double rz = 0.0, sw[3] = {0.0, 0.0, 0.0};
-  // a fixed number of frequency slots per thread, unrolled, so the symbol and
-  // s_hat loads of every slot are in flight together
+  // a fixed number of frequency slots per thread; every symbol and s_hat load
+  // of the thread's slots is issued before the first one is used
   constexpr int EPT = (N * KC + T - 1) / T;
+  constexpr bool FULL = (N * KC) % T == 0;
   const X* __restrict__ shat_d = A.shat ? A.shat + int64_t(design_of(l, A.group)) * 3 * C::NF : nullptr;
   const R* __restrict__ sym = A.sym;
+  auto slot = [&](int k) { return FULL ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, N * KC - 1); };
+  auto freq = [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; };
+  R g[EPT][6];
+  X sh[EPT][3];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int64_t fidx = freq(slot(k));
+#pragma unroll
+    for (int j = 0; j < 6; ++j) g[k][j] = sym[j * C::NF + fidx];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sh[k][c] = c2(R(0), R(0));
+  }
+  if (shat_d) {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int64_t fidx = freq(slot(k));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sh[k][c] = shat_d[int64_t(c) * C::NF + fidx];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = threadIdx.x + k * T;
-    if ((N * KC) % T != 0 && e >= N * KC) break;
+    if (!FULL && e >= N * KC) break;
     const int kcol = e % KC, zz = e / KC;
     const int kx = kx0 + kcol;
-    const int64_t fidx = (int64_t(zz) * N + y) * NKX + kx;
-    X v[3], sh[3];
+    X v[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = cols[(c * N + zz) * KC + kcol];
     if (shat_d) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const R mc = R(mean[c]);
-        sh[c] = shat_d[int64_t(c) * C::NF + fidx];
-        v[c] = c2(fma(-mc, sh[c].x, v[c].x), fma(-mc, sh[c].y, v[c].y));
+        v[c] = c2(fma(-mc, sh[k][c].x, v[c].x), fma(-mc, sh[k][c].y, v[c].y));
       }
     }
-    const R g0 = sym[fidx], g1 = sym[C::NF + fidx], g2 = sym[2 * C::NF + fidx];
-    const R g3 = sym[3 * C::NF + fidx], g4 = sym[4 * C::NF + fidx], g5 = sym[5 * C::NF + fidx];
+    const R g0 = g[k][0], g1 = g[k][1], g2 = g[k][2], g3 = g[k][3], g4 = g[k][4], g5 = g[k][5];
     X w[3];
     w[0] = c2(g0 * v[0].x + g3 * v[1].x + g4 * v[2].x, g0 * v[0].y + g3 * v[1].y + g4 * v[2].y);
     w[1] = c2(g3 * v[0].x + g1 * v[1].x + g5 * v[2].x, g3 * v[0].y + g1 * v[1].y + g5 * v[2].y);
@@ -236,7 +305,7 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
     if (shat_d) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        sw[c] = fma(wgt, double(sh[c].x) * double(w[c].x) + double(sh[c].y) * double(w[c].y), sw[c]);
+        sw[c] = fma(wgt, double(sh[k][c].x) * double(w[c].x) + double(sh[k][c].y) * double(w[c].y), sw[c]);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) cols[(c * N + zz) * KC + kcol] = w[c];
@@ -260,7 +329,8 @@ __global__ __launch_bounds__(Cube<N>::TZ) void fz_static(FFTArgsT<R> A) {
 
 // --------------------------------------------------------------- inverse xy
 template <int N, typename R>
-__global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
+__global__ __launch_bounds__(Cube<N>::TXY) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_xy<R>())))
+void fxyi_static(FFTArgsT<R> A) {
   using C = Cube<N>;
   using X = cplx<R>;
   constexpr int T = C::TXY;
@@ -301,7 +371,7 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
     beta = rz_new / (rz != 0.0 ? rz : 1.0);
   }
   const X* in = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
-  for (int e = threadIdx.x; e < N * C::NKX; e += T) plane[e] = in[e];
+  stage_in<N * C::NKX, T>(plane, in, [](int e) { return e; });
   __syncthreads();
   sfft<true, N, typename C::Cols, T>(plane, A.twy, 1);
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
@@ -334,24 +404,40 @@ __global__ __launch_bounds__(Cube<N>::TXY) void fxyi_static(FFTArgsT<R> A) {
   X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
   const bool scaled = A.s != nullptr;
   const R m = R(mean), tb = R(beta);
-#pragma unroll 4
-  for (int e = 2 * threadIdx.x; e < C::PLANE; e += 2 * T) {
-    const int x = e % N, y = e / N;
-    const X w = plane[y * C::RS + x / 2];
-    X v = w;
+#pragma unroll
+  for (int i0 = 0; i0 < C::ITER; i0 += C::CH) {
+    // every load of the chunk is issued before the first use
+    X vs[C::CH], vp[C::CH], vu[C::CH];
+#pragma unroll
+    for (int i = 0; i < C::CH; ++i) {
+      vs[i] = c2(R(1), R(1));
+      vp[i] = c2(R(0), R(0));
+      vu[i] = c2(R(0), R(0));
+    }
     if (scaled) {
-      const X sv = sp[e / 2];
-      v = c2(sv.x * w.x - m, sv.y * w.y - m);
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
     }
     if (update || move) {
-      const X po = pp[e / 2];
-      if (update) v = c2(fma(tb, po.x, v.x), fma(tb, po.y, v.y));
-      if (move) {
-        const X uo = up[e / 2];
-        up[e / 2] = c2(fma(ra, po.x, uo.x), fma(ra, po.y, uo.y));
-      }
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vp[i] = pp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
     }
-    zp[e / 2] = v;
+    if (move) {
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vu[i] = up[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < C::CH; ++i) {
+      const int h = int(threadIdx.x) + (i0 + i) * T;
+      if (i0 + i >= C::ITER || (!C::EXACT && h >= C::HALF)) continue;
+      const int e = 2 * h;
+      const int x = e % N, y = e / N;
+      const X w = plane[y * C::RS + x / 2];
+      X v = scaled ? c2(vs[i].x * w.x - m, vs[i].y * w.y - m) : w;
+      if (update) v = c2(fma(tb, vp[i].x, v.x), fma(tb, vp[i].y, v.y));
+      zp[h] = v;
+      if (move) up[h] = c2(fma(ra, vp[i].x, vu[i].x), fma(ra, vp[i].y, vu[i].y));
+    }
   }
 }
 
