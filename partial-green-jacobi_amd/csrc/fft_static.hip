@@ -22,9 +22,11 @@ struct Cube {
   static constexpr int KC = N == 16 ? 9 : N == 24 ? 13 : N == 32 ? 17 : N == 48 ? 5 : N == 64 ? 11 : N == 96 ? 7 : 5;
   static constexpr int NKB = NKX / KC;
   static_assert(NKX % KC == 0, "kx chunking must divide the half spectrum");
-  // LDS bytes for complex values of ES bytes, plus a reduction tail of 4 values x 8 waves
-  static constexpr size_t lds_xy(size_t es) { return size_t(N) * RS * es + 4 * 8 * 8; }
-  static constexpr size_t lds_z(size_t es) { return size_t(3) * N * KC * es + 4 * 8 * 8; }
+  // LDS bytes for complex values of ES bytes: the data, the twiddle table W_N^j
+  // (j < N; the same table serves x, y and z on a cube) and a reduction tail of
+  // 4 values x 8 waves
+  static constexpr size_t lds_xy(size_t es) { return (size_t(N) * RS + N) * es + 4 * 8 * 8; }
+  static constexpr size_t lds_z(size_t es) { return (size_t(3) * N * KC + N) * es + 4 * 8 * 8; }
   // Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
   // capped at 4: the register budget the stream batching is compiled for.
   static constexpr int occupancy(size_t lds, int threads) {
@@ -42,28 +44,34 @@ struct Cube {
   // consecutive x, in chunks of CH pairs per thread whose loads are issued together
   static constexpr int HALF = PLANE / 2;
   static constexpr int ITER = (HALF + TXY - 1) / TXY;
-  static constexpr int CH = ITER < 8 ? ITER : 8;
+  static constexpr int CH = ITER <= 8 ? ITER : (ITER % 8 == 0 ? 8 : (ITER + 1) / 2);
   static constexpr bool EXACT = HALF % TXY == 0;
 };
 
-// Global -> LDS copy of COUNT values: every load of a thread is issued before
-// its first LDS store (surplus lanes load a clamped, in-bounds index and store
-// nothing). map(e) is the global index of LDS slot e.
-template <int COUNT, int T, typename X, typename Map>
-__device__ inline void stage_in(X* lds, const X* __restrict__ src, Map map) {
-  constexpr int PER = (COUNT + T - 1) / T;
+// Global -> LDS copy of COUNT values in two phases, so that independent work
+// can run while the loads are in flight: load() issues every load of a thread
+// (surplus lanes load a clamped, in-bounds index), store() writes the LDS slots.
+// map(e) is the global index of LDS slot e.
+template <int COUNT, int T, typename X>
+struct StageIn {
+  static constexpr int PER = (COUNT + T - 1) / T;
   X v[PER];
+  template <typename Map>
+  __device__ inline void load(const X* __restrict__ src, Map map) {
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int e = COUNT % T == 0 ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, COUNT - 1);
-    v[k] = src[map(e)];
+    for (int k = 0; k < PER; ++k) {
+      const int e = COUNT % T == 0 ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, COUNT - 1);
+      v[k] = src[map(e)];
+    }
   }
+  __device__ inline void store(X* lds) const {
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int e = int(threadIdx.x) + k * T;
-    if (COUNT % T == 0 || e < COUNT) lds[e] = v[k];
+    for (int k = 0; k < PER; ++k) {
+      const int e = int(threadIdx.x) + k * T;
+      if (COUNT % T == 0 || e < COUNT) lds[e] = v[k];
+    }
   }
-}
+};
 
 // Pair index of chunk slot i of a vector stream, clamped in bounds for loads.
 template <int N>
@@ -82,7 +90,8 @@ void fxy_static(FFTArgsT<R> A) {
   constexpr int T = C::TXY;
   extern __shared__ __align__(16) unsigned char smem[];
   X* plane = reinterpret_cast<X*>(smem);
-  double* red = reinterpret_cast<double*>(plane + N * C::RS);
+  X* tw = plane + N * C::RS;
+  double* red = reinterpret_cast<double*>(tw + N);
   const int z = blockIdx.x, c = blockIdx.y, l = blockIdx.z;
   const int64_t NN = int64_t(N) * N * N;
   if (MODE == kFxyPCG && !A.st.active[l]) return;
@@ -113,6 +122,8 @@ void fxy_static(FFTArgsT<R> A) {
       }
       vs[i] = c2(R(1), R(1));
     }
+    StageIn<N, T, X> twl;  // the twiddle table, staged with the first chunk
+    if (i0 == 0) twl.load(A.twx, [](int e) { return e; });
     // PCG: the step length is reduced while the r and q loads are in flight
     if (MODE == kFxyPCG && i0 == 0) {
       const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
@@ -125,6 +136,7 @@ void fxy_static(FFTArgsT<R> A) {
         A.st.first[l] = 0;
       }
     }
+    if (i0 == 0) twl.store(tw);
     if (MODE != kFxyScale && scaled) {
 #pragma unroll
       for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
@@ -160,7 +172,7 @@ void fxy_static(FFTArgsT<R> A) {
     }
   }
   __syncthreads();
-  sfft<false, C::M, typename C::Rows, T>(plane, A.twx, 2);
+  sfft<false, C::M, typename C::Rows, T>(plane, tw, 2);
   // split the packed spectrum of every row into its half spectrum
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
   for (int b = threadIdx.x; b < PAIRS; b += T) {
@@ -169,12 +181,12 @@ void fxy_static(FFTArgsT<R> A) {
     const X zk = L[k % C::M], zm = L[(C::M - k) % C::M];
     const X E = c2(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
     const X O = c2(R(0.5) * (zk.y + zm.y), R(-0.5) * (zk.x - zm.x));
-    const X wo = cmul(A.twx[k], O);
+    const X wo = cmul(tw[k], O);
     L[k] = cadd(E, wo);
     if (C::M - k != k) L[C::M - k] = conjd(csub(E, wo));
   }
   __syncthreads();
-  sfft<false, N, typename C::Cols, T>(plane, A.twy, 1);
+  sfft<false, N, typename C::Cols, T>(plane, tw, 1);
   X* out = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
   for (int e = threadIdx.x; e < N * C::NKX; e += T) out[e] = plane[e];
 }
@@ -188,7 +200,8 @@ void fz_static(FFTArgsT<R> A) {
   constexpr int T = C::TZ, KC = C::KC, NKX = C::NKX;
   extern __shared__ __align__(16) unsigned char smem[];
   X* cols = reinterpret_cast<X*>(smem);
-  double* red = reinterpret_cast<double*>(cols + 3 * N * KC);
+  X* tw = cols + 3 * N * KC;
+  double* red = reinterpret_cast<double*>(tw + N);
   // ky is the fastest grid index: the kx chunks of one ky row (which share
   // cache lines) get block ids N apart, a multiple of 8 for every supported N,
   // and so run on the same XCD / L2.
@@ -197,6 +210,29 @@ void fz_static(FFTArgsT<R> A) {
   const int blk = y * C::NKB + kb;
   const bool lead = threadIdx.x == 0 && kb == 0 && y == 0;
   const int64_t NN = int64_t(N) * N * N;
+  // the spectrum columns are loaded first (for every load that may continue)
+  // and stay in flight through the convergence test and the mean reduction
+  if (MODE == kFzPCG && !A.st.active[l]) {
+    // an inactive load stays inactive: the test below with active = false
+    if (lead) {
+      A.st.active_next[l] = 0;
+      A.st.it_next[l] = A.st.it[l];
+      A.st.bad_next[l] = A.st.bad[l];
+      A.st.stopped[l] = 0;
+    }
+    return;
+  }
+  // the spectrum columns and the twiddle table go to LDS first (LDS slot
+  // e = cz * KC + kcol with cz = c * N + zz); they are read after the test below
+  X* S = A.spec + int64_t(l) * 3 * C::NF;
+  {
+    StageIn<3 * N * KC, T, X> in;
+    in.load(S, [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; });
+    StageIn<N, T, X> twl;
+    twl.load(A.twz, [](int e) { return e; });
+    in.store(cols);
+    twl.store(tw);
+  }
   if (MODE == kFzInit || MODE == kFzPCG) {
     const int lane = threadIdx.x & 63;
     double v = 0.0;
@@ -237,11 +273,8 @@ void fz_static(FFTArgsT<R> A) {
     for (int c = 0; c < 3; ++c)
       mean[c] = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
   }
-  X* S = A.spec + int64_t(l) * 3 * C::NF;
-  // LDS slot e = cz * KC + kcol with cz = c * N + zz
-  stage_in<3 * N * KC, T>(cols, S, [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; });
   __syncthreads();
-  sfft<false, N, typename C::ZCols, T>(cols, A.twz, 1);
+  sfft<false, N, typename C::ZCols, T>(cols, tw, 1);
   if (MODE == kFzScale) {
     for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
       const int kcol = e % KC, cz = e / KC;
@@ -279,7 +312,7 @@ void fz_static(FFTArgsT<R> A) {
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = threadIdx.x + k * T;
-    if (!FULL && e >= N * KC) break;
+    if (!FULL && e >= N * KC) continue;
     const int kcol = e % KC, zz = e / KC;
     const int kx = kx0 + kcol;
     X v[3];
@@ -311,7 +344,7 @@ void fz_static(FFTArgsT<R> A) {
     for (int c = 0; c < 3; ++c) cols[(c * N + zz) * KC + kcol] = w[c];
   }
   __syncthreads();
-  sfft<true, N, typename C::ZCols, T>(cols, A.twz, 1);
+  sfft<true, N, typename C::ZCols, T>(cols, tw, 1);
   for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
     const int kcol = e % KC, cz = e / KC;
     S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
@@ -336,6 +369,7 @@ void fxyi_static(FFTArgsT<R> A) {
   constexpr int T = C::TXY;
   extern __shared__ __align__(16) unsigned char smem[];
   X* plane = reinterpret_cast<X*>(smem);
+  X* tw = plane + N * C::RS;
   const int z = blockIdx.x, c = blockIdx.y, l = blockIdx.z;
   const int64_t NN = int64_t(N) * N * N;
   if (threadIdx.x == 0 && z == 0 && c == 0) {
@@ -359,6 +393,11 @@ void fxyi_static(FFTArgsT<R> A) {
     }
     return;
   }
+  // the half spectrum of the plane stays in flight through the mean and beta reductions
+  StageIn<N * C::NKX, T, X> in;
+  StageIn<N, T, X> twl;
+  in.load(A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX, [](int e) { return e; });
+  twl.load(A.twy, [](int e) { return e; });
   const double* pgl = A.pg + int64_t(l) * C::NKB * N * 4;
   double mean = 0.0;
   if (A.shat) mean = wave_reduce_partials(pgl + 1 + c, C::NKB * N, 4) / double(NN);
@@ -370,10 +409,10 @@ void fxyi_static(FFTArgsT<R> A) {
     const double rz = A.st.rz[l];
     beta = rz_new / (rz != 0.0 ? rz : 1.0);
   }
-  const X* in = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
-  stage_in<N * C::NKX, T>(plane, in, [](int e) { return e; });
+  in.store(plane);
+  twl.store(tw);
   __syncthreads();
-  sfft<true, N, typename C::Cols, T>(plane, A.twy, 1);
+  sfft<true, N, typename C::Cols, T>(plane, tw, 1);
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
   for (int b = threadIdx.x; b < PAIRS; b += T) {
     const int row = b / (HALF + 1), k = b % (HALF + 1);
@@ -383,12 +422,12 @@ void fxyi_static(FFTArgsT<R> A) {
       xk.y = R(0);
       xm.y = R(0);
     }
-    const X wk = conjd(A.twx[k]);
+    const X wk = conjd(tw[k]);
     const X e1 = c2(xk.x + xm.x, xk.y - xm.y);
     const X o1 = cmul(c2(xk.x - xm.x, xk.y + xm.y), wk);
     const X zk = c2(e1.x - o1.y, e1.y + o1.x);
     if (k != 0 && C::M - k != k) {
-      const X wm = conjd(A.twx[C::M - k]);
+      const X wm = conjd(tw[C::M - k]);
       const X e2 = c2(xm.x + xk.x, xm.y - xk.y);
       const X o2 = cmul(c2(xm.x - xk.x, xm.y + xk.y), wm);
       L[C::M - k] = c2(e2.x - o2.y, e2.y + o2.x);
@@ -396,7 +435,7 @@ void fxyi_static(FFTArgsT<R> A) {
     L[k] = zk;
   }
   __syncthreads();
-  sfft<true, C::M, typename C::Rows, T>(plane, A.twx, 2);
+  sfft<true, C::M, typename C::Rows, T>(plane, tw, 2);
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
   const X* __restrict__ pp = A.p != nullptr ? reinterpret_cast<const X*>(A.p + off) : nullptr;
