@@ -273,6 +273,39 @@ void fz_static(FFTArgsT<R> A) {
     for (int c = 0; c < 3; ++c)
       mean[c] = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
   }
+  // a fixed number of frequency slots per thread; every symbol and s_hat load
+  // of the thread's slots is issued before the first is used. In FP32 they are
+  // issued before the forward transform and stay in flight while it runs; FP64
+  // lacks the registers for that and issues them after it (the scaling pass
+  // needs none)
+  constexpr bool EARLY = sizeof(R) == 4;
+  constexpr int EPT = (N * KC + T - 1) / T;
+  constexpr bool FULL = (N * KC) % T == 0;
+  const X* __restrict__ shat_d = A.shat ? A.shat + int64_t(design_of(l, A.group)) * 3 * C::NF : nullptr;
+  const R* __restrict__ sym = A.sym;
+  auto slot = [&](int k) { return FULL ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, N * KC - 1); };
+  auto freq = [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; };
+  R g[EPT][6];
+  X sh[EPT][3];
+  auto load_green = [&]() {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int64_t fidx = freq(slot(k));
+#pragma unroll
+      for (int j = 0; j < 6; ++j) g[k][j] = sym[j * C::NF + fidx];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sh[k][c] = c2(R(0), R(0));
+    }
+    if (shat_d) {
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int64_t fidx = freq(slot(k));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sh[k][c] = shat_d[int64_t(c) * C::NF + fidx];
+      }
+    }
+  };
+  if (MODE != kFzScale && EARLY) load_green();
   __syncthreads();
   sfft<false, N, typename C::ZCols, T>(cols, tw, 1);
   if (MODE == kFzScale) {
@@ -282,33 +315,8 @@ void fz_static(FFTArgsT<R> A) {
     }
     return;
   }
+  if (!EARLY) load_green();
   double rz = 0.0, sw[3] = {0.0, 0.0, 0.0};
-  // a fixed number of frequency slots per thread; every symbol and s_hat load
-  // of the thread's slots is issued before the first one is used
-  constexpr int EPT = (N * KC + T - 1) / T;
-  constexpr bool FULL = (N * KC) % T == 0;
-  const X* __restrict__ shat_d = A.shat ? A.shat + int64_t(design_of(l, A.group)) * 3 * C::NF : nullptr;
-  const R* __restrict__ sym = A.sym;
-  auto slot = [&](int k) { return FULL ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, N * KC - 1); };
-  auto freq = [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; };
-  R g[EPT][6];
-  X sh[EPT][3];
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int64_t fidx = freq(slot(k));
-#pragma unroll
-    for (int j = 0; j < 6; ++j) g[k][j] = sym[j * C::NF + fidx];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) sh[k][c] = c2(R(0), R(0));
-  }
-  if (shat_d) {
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int64_t fidx = freq(slot(k));
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sh[k][c] = shat_d[int64_t(c) * C::NF + fidx];
-    }
-  }
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = threadIdx.x + k * T;
@@ -411,6 +419,40 @@ void fxyi_static(FFTArgsT<R> A) {
   }
   in.store(plane);
   twl.store(tw);
+  // output streams: every load of a chunk is issued before the first use; in
+  // FP32 the first chunk is issued before the inverse transforms and stays in
+  // flight while they run (FP64, and planes of more than one chunk, lack the
+  // registers for that)
+  constexpr bool EARLY = sizeof(R) == 4 && C::ITER == C::CH;
+  const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
+  const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
+  const X* __restrict__ pp = A.p != nullptr ? reinterpret_cast<const X*>(A.p + off) : nullptr;
+  X* __restrict__ up = move ? reinterpret_cast<X*>(A.u + off) : nullptr;
+  X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
+  const bool scaled = A.s != nullptr;
+  const R m = R(mean), tb = R(beta);
+  X vs[C::CH], vp[C::CH], vu[C::CH];
+  auto load_chunk = [&](int i0) {
+#pragma unroll
+    for (int i = 0; i < C::CH; ++i) {
+      vs[i] = c2(R(1), R(1));
+      vp[i] = c2(R(0), R(0));
+      vu[i] = c2(R(0), R(0));
+    }
+    if (scaled) {
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+    }
+    if (update || move) {
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vp[i] = pp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+    }
+    if (move) {
+#pragma unroll
+      for (int i = 0; i < C::CH; ++i) vu[i] = up[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+    }
+  };
+  if (EARLY) load_chunk(0);
   __syncthreads();
   sfft<true, N, typename C::Cols, T>(plane, tw, 1);
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
@@ -436,35 +478,9 @@ void fxyi_static(FFTArgsT<R> A) {
   }
   __syncthreads();
   sfft<true, C::M, typename C::Rows, T>(plane, tw, 2);
-  const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
-  const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
-  const X* __restrict__ pp = A.p != nullptr ? reinterpret_cast<const X*>(A.p + off) : nullptr;
-  X* __restrict__ up = move ? reinterpret_cast<X*>(A.u + off) : nullptr;
-  X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
-  const bool scaled = A.s != nullptr;
-  const R m = R(mean), tb = R(beta);
 #pragma unroll
   for (int i0 = 0; i0 < C::ITER; i0 += C::CH) {
-    // every load of the chunk is issued before the first use
-    X vs[C::CH], vp[C::CH], vu[C::CH];
-#pragma unroll
-    for (int i = 0; i < C::CH; ++i) {
-      vs[i] = c2(R(1), R(1));
-      vp[i] = c2(R(0), R(0));
-      vu[i] = c2(R(0), R(0));
-    }
-    if (scaled) {
-#pragma unroll
-      for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
-    }
-    if (update || move) {
-#pragma unroll
-      for (int i = 0; i < C::CH; ++i) vp[i] = pp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
-    }
-    if (move) {
-#pragma unroll
-      for (int i = 0; i < C::CH; ++i) vu[i] = up[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
-    }
+    if (!(EARLY && i0 == 0)) load_chunk(i0);
 #pragma unroll
     for (int i = 0; i < C::CH; ++i) {
       const int h = int(threadIdx.x) + (i0 + i) * T;
