@@ -95,16 +95,6 @@ __device__ __forceinline__ double reduce_pr_sum(const FFTArgs& A, int l, int c) 
   return wave_reduce_partials(A.pr + (int64_t(l) * A.nz * 3 + c) * 2 + 1, A.nz, 6);
 }
 
-__device__ __forceinline__ double reduce_pr_sq(const FFTArgs& A, int l) {
-  const int lane = threadIdx.x & 63;
-  double v = 0.0;
-  for (int i = lane; i < A.nz; i += 64) {
-    const double* p = A.pr + (int64_t(l) * A.nz + i) * 6;
-    v += (p[0] + p[2]) + p[4];
-  }
-  return wave_sum(v);
-}
-
 // --------------------------------------------------------------- forward xy
 template <int MODE>
 __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
@@ -121,13 +111,15 @@ __global__ __launch_bounds__(512) void fxy_kernel(FFTArgs A) {
   double alpha = 0.0;
   if (MODE == kFxyPCG) {
     if (!A.st.active[l]) return;
-    const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
-    const double rz = wave_reduce_partials(A.pg + int64_t(l) * f.nkb * ny * 4, f.nkb * ny, 4);
+    double pq, rz;
+    wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * f.nkb * ny * 4,
+                           f.nkb * ny, 4, pq, rz);
     const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
     alpha = valid ? rz / pq : 0.0;
     if (threadIdx.x == 0 && z == 0 && blockIdx.y == 0) {
       A.st.rz[l] = rz;
       A.st.alpha[l] = alpha;
+      A.st.valid[l] = valid;
       A.st.first[l] = 0;
     }
   }
@@ -221,8 +213,21 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
   double* red = reinterpret_cast<double*>(cols + size_t(kc) * 3 * nz);
   double* pg = A.pg + (int64_t(l) * f.nkb * ny + blk) * 4;
 
+  double mean[3] = {0.0, 0.0, 0.0};
   if (MODE == kFzInit || MODE == kFzPCG) {
-    const double res2 = reduce_pr_sq(A, l);
+    // one pass over the per-plane partials (|r|^2 and the component sums of r)
+    const int lane = threadIdx.x & 63;
+    double v = 0.0, vm[3] = {0.0, 0.0, 0.0};
+    for (int i = lane; i < A.nz; i += 64) {
+      const double* p = A.pr + (int64_t(l) * A.nz + i) * 6;
+      v += (p[0] + p[2]) + p[4];
+      vm[0] += p[1];
+      vm[1] += p[3];
+      vm[2] += p[5];
+    }
+    const double res2 = wave_sum(v);
+    if (A.shat)
+      for (int c = 0; c < 3; ++c) mean[c] = wave_sum(vm[c]) / double(A.N);
     bool act;
     if (MODE == kFzInit) {
       act = res2 > A.st.target2[l] && A.st.it[l] < kCap && !A.st.bad[l];
@@ -235,9 +240,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
       }
     } else {
       const bool active = A.st.active[l] != 0;
-      const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
-      const double rz = A.st.rz[l];
-      const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
+      const bool valid = A.st.valid[l] != 0;  // set by the forward pass of this iteration
       const bool finite = isfinite(res2);
       const int64_t it = A.st.it[l] + (active ? 1 : 0);
       act = active && valid && finite && res2 > A.st.target2[l] && it < kCap;
@@ -249,9 +252,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
       }
     }
     if (!act) return;
-  }
-  double mean[3] = {0.0, 0.0, 0.0};
-  if (MODE != kFzScale && A.shat) {
+  } else if (MODE != kFzScale && A.shat) {
     for (int c = 0; c < 3; ++c) mean[c] = reduce_pr_sum(A, l, c) / double(A.N);
   }
   // load the columns (c, z, kcol) of this (ky, kx-chunk)
@@ -354,20 +355,22 @@ __global__ __launch_bounds__(512) void fxyi_kernel(FFTArgs A) {
       return;
     }
   }
-  double mean[3] = {0.0, 0.0, 0.0};
-  if (A.shat) {
-    const int cnt = f.nkb * ny;
-    for (int c = 0; c < 3; ++c)
-      mean[c] = wave_reduce_partials(A.pg + int64_t(l) * cnt * 4 + 1 + c, cnt, 4) / double(A.N);
-  }
   // PCG: the output is the next direction p = z + beta p (z alone right after init)
+  double mean[3] = {0.0, 0.0, 0.0};
   double beta = 0.0;
   const bool update = MODE == kFxyiPCG && A.p != nullptr && !A.st.first[l];
-  if (update) {
+  if (A.shat || update) {
+    // one pass: r.z (offset 0) and the s.w partials of the three components (offsets 1..3)
     const int cnt = f.nkb * ny;
-    const double rz_new = wave_reduce_partials(A.pg + int64_t(l) * cnt * 4, cnt, 4);
-    const double rz = A.st.rz[l];
-    beta = rz_new / (rz != 0.0 ? rz : 1.0);
+    const int off[4] = {0, 1, 2, 3};
+    double sums[4];
+    wave_reduce_partials_n<4>(A.pg + int64_t(l) * cnt * 4, off, cnt, 4, sums);
+    if (A.shat)
+      for (int c = 0; c < 3; ++c) mean[c] = sums[1 + c] / double(A.N);
+    if (update) {
+      const double rz = A.st.rz[l];
+      beta = sums[0] / (rz != 0.0 ? rz : 1.0);
+    }
   }
   // 1. load the half spectrum of the plane
   const double2* in = A.spec + ((int64_t(l) * 3 + c0) * A.nz + z) * int64_t(ny) * f.nkx;

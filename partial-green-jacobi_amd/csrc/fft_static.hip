@@ -126,13 +126,15 @@ void fxy_static(FFTArgsT<R> A) {
     if (i0 == 0) twl.load(A.twx, [](int e) { return e; });
     // PCG: the step length is reduced while the r and q loads are in flight
     if (MODE == kFxyPCG && i0 == 0) {
-      const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
-      const double rz = wave_reduce_partials(A.pg + int64_t(l) * C::NKB * N * 4, C::NKB * N, 4);
+      double pq, rz;
+      wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * C::NKB * N * 4,
+                             C::NKB * N, 4, pq, rz);
       const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
       alpha = valid ? rz / pq : 0.0;
       if (threadIdx.x == 0 && z == 0 && c == 0) {
         A.st.rz[l] = rz;
         A.st.alpha[l] = alpha;
+        A.st.valid[l] = valid;
         A.st.first[l] = 0;
       }
     }
@@ -233,14 +235,21 @@ void fz_static(FFTArgsT<R> A) {
     in.store(cols);
     twl.store(tw);
   }
+  double mean[3] = {0.0, 0.0, 0.0};
   if (MODE == kFzInit || MODE == kFzPCG) {
     const int lane = threadIdx.x & 63;
-    double v = 0.0;
+    // one pass over the per-plane partials (|r|^2 and the component sums of r)
+    double v = 0.0, vm[3] = {0.0, 0.0, 0.0};
     for (int i = lane; i < N; i += 64) {
       const double* p = A.pr + (int64_t(l) * N + i) * 6;
       v += (p[0] + p[2]) + p[4];
+      vm[0] += p[1];
+      vm[1] += p[3];
+      vm[2] += p[5];
     }
     const double res2 = wave_sum(v);
+    if (A.shat)
+      for (int c = 0; c < 3; ++c) mean[c] = wave_sum(vm[c]) / double(NN);
     bool act;
     if (MODE == kFzInit) {
       act = res2 > A.st.target2[l] && A.st.it[l] < kCap && !A.st.bad[l];
@@ -253,9 +262,7 @@ void fz_static(FFTArgsT<R> A) {
       }
     } else {
       const bool active = A.st.active[l] != 0;
-      const double pq = wave_reduce_partials(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1);
-      const double rz = A.st.rz[l];
-      const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
+      const bool valid = A.st.valid[l] != 0;  // set by the forward pass of this iteration
       const bool finite = isfinite(res2);
       const int64_t it = A.st.it[l] + (active ? 1 : 0);
       act = active && valid && finite && res2 > A.st.target2[l] && it < kCap;
@@ -267,9 +274,7 @@ void fz_static(FFTArgsT<R> A) {
       }
     }
     if (!act) return;
-  }
-  double mean[3] = {0.0, 0.0, 0.0};
-  if (MODE != kFzScale && A.shat) {
+  } else if (MODE != kFzScale && A.shat) {
     for (int c = 0; c < 3; ++c)
       mean[c] = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
   }
@@ -407,15 +412,19 @@ void fxyi_static(FFTArgsT<R> A) {
   in.load(A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX, [](int e) { return e; });
   twl.load(A.twy, [](int e) { return e; });
   const double* pgl = A.pg + int64_t(l) * C::NKB * N * 4;
-  double mean = 0.0;
-  if (A.shat) mean = wave_reduce_partials(pgl + 1 + c, C::NKB * N, 4) / double(NN);
   // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
   const bool update = A.p != nullptr && !A.st.first[l];
-  double beta = 0.0;
-  if (update) {
-    const double rz_new = wave_reduce_partials(pgl, C::NKB * N, 4);
-    const double rz = A.st.rz[l];
-    beta = rz_new / (rz != 0.0 ? rz : 1.0);
+  double mean = 0.0, beta = 0.0;
+  if (A.shat || update) {
+    // one pass: r.z (offset 0) and this component's s.w (offset 1 + c) partials
+    const int off[2] = {0, 1 + c};
+    double sums[2];
+    wave_reduce_partials_n<2>(pgl, off, C::NKB * N, 4, sums);
+    if (A.shat) mean = sums[1] / double(NN);
+    if (update) {
+      const double rz = A.st.rz[l];
+      beta = sums[0] / (rz != 0.0 ? rz : 1.0);
+    }
   }
   in.store(plane);
   twl.store(tw);

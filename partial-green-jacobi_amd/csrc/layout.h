@@ -274,7 +274,7 @@ constexpr int kMaxResidentLoads = 4096;
 // Resident state: public blocks first (documented in pgj.h), then private.
 struct StateLayout {
   size_t rz, target2, norm, absolute, it, active, bad;
-  size_t it_next, active_next, bad_next, first, alpha, stopped;
+  size_t it_next, active_next, bad_next, first, alpha, stopped, valid;
   size_t bytes;
 };
 
@@ -294,6 +294,7 @@ inline StateLayout state_layout(int L) {
   s.first = off; off += align8(size_t(L));
   s.alpha = off; off += align8(size_t(L) * 8);
   s.stopped = off; off += align8(size_t(L));
+  s.valid = off; off += align8(size_t(L));
   s.bytes = align256(off);
   return s;
 }
@@ -313,6 +314,7 @@ struct StateView {
   uint8_t* first;
   double* alpha;
   uint8_t* stopped;   // active at the start of this iteration and stopped by its test
+  uint8_t* valid;     // this iteration's p.q and r.z were finite and positive (forward pass)
 };
 
 // View of loads [l0, ...) of a state block laid out for L loads.
@@ -333,6 +335,7 @@ inline StateView state_view(void* base, int L, int l0 = 0) {
   v.first = reinterpret_cast<uint8_t*>(b + s.first) + l0;
   v.alpha = reinterpret_cast<double*>(b + s.alpha) + l0;
   v.stopped = reinterpret_cast<uint8_t*>(b + s.stopped) + l0;
+  v.valid = reinterpret_cast<uint8_t*>(b + s.valid) + l0;
   return v;
 }
 

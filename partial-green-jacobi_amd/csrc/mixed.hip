@@ -36,6 +36,7 @@ __global__ void inner_scale(StateView outer, StateView inner, double* rscale, do
   inner.active[l] = 0;
   inner.first[l] = 0;
   inner.stopped[l] = 0;
+  inner.valid[l] = 0;
 }
 
 __global__ __launch_bounds__(256) void inner_convert(const double* r64, const double* rscale, float* r32, float* u32,

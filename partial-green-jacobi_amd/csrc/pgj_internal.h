@@ -98,4 +98,39 @@ __device__ __forceinline__ double wave_reduce_partials(const double* p, int coun
   return wave_sum(v);
 }
 
+// K reductions of `count` entries of stride `stride` in one lane-strided pass
+// (reduction k reads p[i * stride + off[k]]), so their loads are in flight
+// together. Each accumulates in the order of wave_reduce_partials, so the
+// results are bit-identical to K separate calls.
+template <int K>
+__device__ __forceinline__ void wave_reduce_partials_n(const double* p, const int (&off)[K], int count, int stride,
+                                                       double (&out)[K]) {
+  const int lane = threadIdx.x & 63;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  for (int i = lane; i < count; i += 64) {
+    const double* q = p + (int64_t)i * stride;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += q[off[k]];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = wave_sum(v[k]);
+}
+
+// Two reductions over different partial arrays in one pass (bit-identical to
+// two wave_reduce_partials calls).
+__device__ __forceinline__ void wave_reduce_partials_2(const double* p, int pc, int ps, const double* q, int qc, int qs,
+                                                       double& a, double& b) {
+  const int lane = threadIdx.x & 63;
+  double va = 0.0, vb = 0.0;
+  const int n = pc > qc ? pc : qc;
+  for (int i = lane; i < n; i += 64) {
+    if (i < pc) va += p[(int64_t)i * ps];
+    if (i < qc) vb += q[(int64_t)i * qs];
+  }
+  a = wave_sum(va);
+  b = wave_sum(vb);
+}
+
 }  // namespace pgj

@@ -41,6 +41,7 @@ __global__ void setup_finish(const double* part, int nblk, double rtol, double a
     st.active[l] = 0;
     st.first[l] = 0;
     st.stopped[l] = 0;
+    st.valid[l] = 0;
   }
 }
 
