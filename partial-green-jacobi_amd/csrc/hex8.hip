@@ -40,7 +40,13 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   const int W = c.W;
   const int nx = A.nx, ny = A.ny, nz = A.nz;
   const unsigned N = unsigned(A.N);
-  const int y0 = blockIdx.x * W * R, z0 = blockIdx.y * c.ZC;
+  // XCD-aware row groups: workgroups are dealt to the 8 XCDs round robin by
+  // linear id, so when the row-group count is a multiple of 8 each XCD gets a
+  // contiguous range of row groups, and the halo row a block reads has just
+  // been fetched into the same L2 by its neighbour.
+  const int nrg = int(gridDim.x);
+  const int rg = nrg % 8 == 0 ? int(blockIdx.x % 8) * (nrg / 8) + int(blockIdx.x / 8) : int(blockIdx.x);
+  const int y0 = rg * W * R, z0 = blockIdx.y * c.ZC;
   const int xl = lane % c.nl;
   const bool lane_on = lane < c.nl;
   const unsigned xo = unsigned(xl * XPL);
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
     }
     if (MODE == kModePCG) {
       const double total = block_sum(pq, reinterpret_cast<double*>(lds + 2 * slot_size));
-      if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
+      if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + rg] = total;
     }
     return;
   } else {
@@ -391,7 +397,7 @@ __global__ __launch_bounds__(MAXT) void hex8_rowmarch(RowArgsT<T> A) {
   }
   if (MODE == kModePCG) {
     const double total = block_sum(pq, reinterpret_cast<double*>(lds + 2 * slot_size));
-    if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
+    if (threadIdx.x == 0) A.ppq[int64_t(l) * A.ppq_stride + blockIdx.y * gridDim.x + rg] = total;
   }
   }
 }
