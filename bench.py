@@ -352,6 +352,7 @@ def serve_form(args):
     done = server.run(clients, arrivals, horizon=args.serve_seconds, deadline=2 * args.serve_seconds)
     elapsed = server.elapsed
     server_unserved, server_slots = server.unserved, {str(n): g.D for n, g in server.groups.items()}
+    phases = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in server.phases.items()}
     resident = sum(server.memory().values()) / 2 ** 30
     del server
     torch.cuda.empty_cache()
@@ -381,6 +382,7 @@ def serve_form(args):
                                         'p99': 1e3 * float(np.percentile(v, 99)), 'count': len(v)}
                                for n, v in sorted(by_size.items())},
         'unserved': unserved, 'max_relative_residual': rel, 'slots': slots, 'resident_gib': resident,
+        'server_phases_s': phases,
         'plain_sequential': {'value': len(base_done) / base.elapsed, 'completed': len(base_done),
                              'unserved': base.unserved, 'p50_ms': 1e3 * base_stats['p50'],
                              'p99_ms': 1e3 * base_stats['p99']},

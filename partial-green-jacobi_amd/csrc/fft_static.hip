@@ -10,6 +10,13 @@
 
 namespace pgj {
 
+// Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
+// capped at 4: the register budget the stream batching is compiled for.
+constexpr int lds_occupancy(size_t lds, int threads) {
+  const int w = int(size_t(160 * 1024) / lds) * (threads / 64) / 4;
+  return w < 1 ? 1 : (w > 4 ? 4 : w);
+}
+
 template <int N>
 struct Cube {
   static constexpr int M = N / 2;             // complex length of the packed x transform
@@ -18,34 +25,44 @@ struct Cube {
   static constexpr int PLANE = N * N;         // nodes per z-plane
   static constexpr int64_t NF = int64_t(N) * N * NKX;
   static constexpr int TXY = N >= 96 ? 512 : 256;
-  static constexpr int TZ = 256;
-  static constexpr int KC = N == 16 ? 9 : N == 24 ? 13 : N == 32 ? 17 : N == 48 ? 5 : N == 64 ? 11 : N == 96 ? 7 : 5;
-  static constexpr int NKB = NKX / KC;
-  static_assert(NKX % KC == 0, "kx chunking must divide the half spectrum");
   // LDS bytes for complex values of ES bytes: the data, the twiddle table W_N^j
-  // (j < N; the same table serves x, y and z on a cube) and a reduction tail of
-  // 4 values x 8 waves
+  // (j < N; the same table serves x, y and z on a cube) and a reduction tail
+  // of 4 values x 8 waves
   static constexpr size_t lds_xy(size_t es) { return (size_t(N) * RS + N) * es + 4 * 8 * 8; }
-  static constexpr size_t lds_z(size_t es) { return (size_t(3) * N * KC + N) * es + 4 * 8 * 8; }
-  // Waves per SIMD that the LDS of a block allows (160 KiB per CU, 4 SIMDs),
-  // capped at 4: the register budget the stream batching is compiled for.
-  static constexpr int occupancy(size_t lds, int threads) {
-    const int w = int(size_t(160 * 1024) / lds) * (threads / 64) / 4;
-    return w < 1 ? 1 : (w > 4 ? 4 : w);
-  }
   template <typename R>
-  static constexpr int occ_xy() { return occupancy(lds_xy(2 * sizeof(R)), TXY); }
-  template <typename R>
-  static constexpr int occ_z() { return occupancy(lds_z(2 * sizeof(R)), TZ); }
+  static constexpr int occ_xy() { return lds_occupancy(lds_xy(2 * sizeof(R)), TXY); }
   using Rows = Lines<N, N, 0, RS, 1, false>;          // x transforms (length M) on the N rows
   using Cols = Lines<NKX, NKX, 0, 1, RS, true>;       // y transforms (length N) on the NKX columns
-  using ZCols = Lines<3 * KC, KC, N * KC, 1, KC, true>;  // z transforms on (component, column)
   // vector streams of one z-plane: pairs h = threadIdx.x + i TXY (i < ITER) of
   // consecutive x, in chunks of CH pairs per thread whose loads are issued together
   static constexpr int HALF = PLANE / 2;
   static constexpr int ITER = (HALF + TXY - 1) / TXY;
   static constexpr int CH = ITER <= 8 ? ITER : (ITER % 8 == 0 ? 8 : (ITER + 1) / 2);
   static constexpr bool EXACT = HALF % TXY == 0;
+};
+
+// z pass of a cube in precision R: KC kx columns of all three components per
+// block; a row of KC complex values is the contiguous run a block reads per
+// (c, kz). The last chunk is partial when KC does not divide NKX (96: 17 + 17
+// + 15). 128^3 FP64 keeps 5 columns: 13 would need 147 VGPRs at 2 blocks per CU.
+template <int N, typename R>
+struct ZPass {
+  static constexpr int NKX = N / 2 + 1;
+  static constexpr int TZ = N >= 96 ? 512 : 256;
+  static constexpr int KC = N == 16    ? 9
+                            : N == 24  ? 13
+                            : N == 32  ? 17
+                            : N == 48  ? 5
+                            : N == 64  ? 11
+                            : N == 96  ? 17
+                            : sizeof(R) == 8 ? 5 : 13;
+  static constexpr int NKB = (NKX + KC - 1) / KC;
+  static constexpr bool KFULL = NKX % KC == 0;
+  // columns and the twiddle table; the reduction reuses the column storage once
+  // the columns are written back (2 blocks of the 128^3 FP32 pass fit one CU)
+  static constexpr size_t lds = (size_t(3) * N * KC + N) * 2 * sizeof(R);
+  static constexpr int occ = lds_occupancy(lds, TZ);
+  using ZCols = Lines<3 * KC, KC, N * KC, 1, KC, true>;  // z transforms on (component, column)
 };
 
 // Global -> LDS copy of COUNT values in two phases, so that independent work
@@ -127,8 +144,9 @@ void fxy_static(FFTArgsT<R> A) {
     // PCG: the step length is reduced while the r and q loads are in flight
     if (MODE == kFxyPCG && i0 == 0) {
       double pq, rz;
-      wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * C::NKB * N * 4,
-                             C::NKB * N, 4, pq, rz);
+      constexpr int NKB = ZPass<N, R>::NKB;
+      wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * NKB * N * 4,
+                             NKB * N, 4, pq, rz);
       const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
       alpha = valid ? rz / pq : 0.0;
       if (threadIdx.x == 0 && z == 0 && c == 0) {
@@ -195,21 +213,22 @@ void fxy_static(FFTArgsT<R> A) {
 
 // ---------------------------------------------------------------- z + Green
 template <int N, int MODE, typename R>
-__global__ __launch_bounds__(Cube<N>::TZ) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_z<R>())))
+__global__ __launch_bounds__((ZPass<N, R>::TZ)) __attribute__((amdgpu_waves_per_eu((ZPass<N, R>::occ))))
 void fz_static(FFTArgsT<R> A) {
   using C = Cube<N>;
   using X = cplx<R>;
-  constexpr int T = C::TZ, KC = C::KC, NKX = C::NKX;
+  using Z = ZPass<N, R>;
+  constexpr int T = Z::TZ, KC = Z::KC, NKX = C::NKX;
   extern __shared__ __align__(16) unsigned char smem[];
   X* cols = reinterpret_cast<X*>(smem);
   X* tw = cols + 3 * N * KC;
-  double* red = reinterpret_cast<double*>(tw + N);
+  double* red = reinterpret_cast<double*>(cols);  // used only after the columns are written back
   // ky is the fastest grid index: the kx chunks of one ky row (which share
   // cache lines) get block ids N apart, a multiple of 8 for every supported N,
   // and so run on the same XCD / L2.
   const int y = blockIdx.x, kb = blockIdx.y, l = blockIdx.z;
   const int kx0 = kb * KC;
-  const int blk = y * C::NKB + kb;
+  const int blk = y * Z::NKB + kb;
   const bool lead = threadIdx.x == 0 && kb == 0 && y == 0;
   const int64_t NN = int64_t(N) * N * N;
   // the spectrum columns are loaded first (for every load that may continue)
@@ -227,9 +246,16 @@ void fz_static(FFTArgsT<R> A) {
   // the spectrum columns and the twiddle table go to LDS first (LDS slot
   // e = cz * KC + kcol with cz = c * N + zz); they are read after the test below
   X* S = A.spec + int64_t(l) * 3 * C::NF;
+  // columns of this chunk (the last chunk may hold fewer than KC; its missing columns are zero in LDS)
+  const int kcn = Z::KFULL ? KC : min(KC, NKX - kx0);
   {
     StageIn<3 * N * KC, T, X> in;
-    in.load(S, [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; });
+    in.load(S, [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + min(e % KC, kcn - 1); });
+    if (!Z::KFULL) {
+#pragma unroll
+      for (int k = 0; k < in.PER; ++k)
+        if ((int(threadIdx.x) + k * T) % KC >= kcn) in.v[k] = c2(R(0), R(0));
+    }
     StageIn<N, T, X> twl;
     twl.load(A.twz, [](int e) { return e; });
     in.store(cols);
@@ -278,94 +304,101 @@ void fz_static(FFTArgsT<R> A) {
     for (int c = 0; c < 3; ++c)
       mean[c] = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
   }
-  // a fixed number of frequency slots per thread; every symbol and s_hat load
-  // of the thread's slots is issued before the first is used. In FP32 they are
-  // issued before the forward transform and stay in flight while it runs; FP64
-  // lacks the registers for that and issues them after it (the scaling pass
-  // needs none)
-  constexpr bool EARLY = sizeof(R) == 4;
+  // a fixed number of frequency slots per thread, in groups of GS slots whose
+  // symbol and s_hat loads are all issued before the first is used. In FP32 one
+  // group holds every slot and is issued before the forward transform, so the
+  // loads stay in flight while it runs; FP64, and FP32 with more than 3 slots,
+  // lack the registers for that and issue their groups after it (the scaling
+  // pass needs none).
   constexpr int EPT = (N * KC + T - 1) / T;
+  constexpr bool EARLY = sizeof(R) == 4 && EPT <= 3;
+  constexpr int GS = (EARLY || EPT <= 3) ? EPT : 2;
   constexpr bool FULL = (N * KC) % T == 0;
   const X* __restrict__ shat_d = A.shat ? A.shat + int64_t(design_of(l, A.group)) * 3 * C::NF : nullptr;
   const R* __restrict__ sym = A.sym;
   auto slot = [&](int k) { return FULL ? int(threadIdx.x) + k * T : min(int(threadIdx.x) + k * T, N * KC - 1); };
-  auto freq = [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + e % KC; };
-  R g[EPT][6];
-  X sh[EPT][3];
-  auto load_green = [&]() {
+  auto freq = [&](int e) { return (int64_t(e / KC) * N + y) * NKX + kx0 + min(e % KC, kcn - 1); };
+  R g[GS][6];
+  X sh[GS][3];
+  auto load_group = [&](int k0) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int64_t fidx = freq(slot(k));
+    for (int i = 0; i < GS; ++i) {
+      const int64_t fidx = freq(slot(k0 + i < EPT ? k0 + i : EPT - 1));
 #pragma unroll
-      for (int j = 0; j < 6; ++j) g[k][j] = sym[j * C::NF + fidx];
+      for (int j = 0; j < 6; ++j) g[i][j] = sym[j * C::NF + fidx];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sh[k][c] = c2(R(0), R(0));
+      for (int c = 0; c < 3; ++c) sh[i][c] = c2(R(0), R(0));
     }
     if (shat_d) {
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int64_t fidx = freq(slot(k));
+      for (int i = 0; i < GS; ++i) {
+        const int64_t fidx = freq(slot(k0 + i < EPT ? k0 + i : EPT - 1));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) sh[k][c] = shat_d[int64_t(c) * C::NF + fidx];
+        for (int c = 0; c < 3; ++c) sh[i][c] = shat_d[int64_t(c) * C::NF + fidx];
       }
     }
   };
-  if (MODE != kFzScale && EARLY) load_green();
+  if (MODE != kFzScale && EARLY) load_group(0);
   __syncthreads();
-  sfft<false, N, typename C::ZCols, T>(cols, tw, 1);
+  sfft<false, N, typename Z::ZCols, T>(cols, tw, 1);
   if (MODE == kFzScale) {
     for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
       const int kcol = e % KC, cz = e / KC;
-      S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
+      if (Z::KFULL || kcol < kcn) S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
     }
     return;
   }
-  if (!EARLY) load_green();
   double rz = 0.0, sw[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int e = threadIdx.x + k * T;
-    if (!FULL && e >= N * KC) continue;
-    const int kcol = e % KC, zz = e / KC;
-    const int kx = kx0 + kcol;
-    X v[3];
+  for (int k0 = 0; k0 < EPT; k0 += GS) {
+    if (!(EARLY && k0 == 0)) load_group(k0);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = cols[(c * N + zz) * KC + kcol];
-    if (shat_d) {
+    for (int i = 0; i < GS; ++i) {
+      const int k = k0 + i;
+      const int e = threadIdx.x + k * T;
+      if (k >= EPT || (!FULL && e >= N * KC)) continue;
+      const int kcol = e % KC, zz = e / KC;
+      if (!Z::KFULL && kcol >= kcn) continue;
+      const int kx = kx0 + kcol;
+      X v[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const R mc = R(mean[c]);
-        v[c] = c2(fma(-mc, sh[k][c].x, v[c].x), fma(-mc, sh[k][c].y, v[c].y));
+      for (int c = 0; c < 3; ++c) v[c] = cols[(c * N + zz) * KC + kcol];
+      if (shat_d) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const R mc = R(mean[c]);
+          v[c] = c2(fma(-mc, sh[i][c].x, v[c].x), fma(-mc, sh[i][c].y, v[c].y));
+        }
       }
+      const R g0 = g[i][0], g1 = g[i][1], g2 = g[i][2], g3 = g[i][3], g4 = g[i][4], g5 = g[i][5];
+      X w[3];
+      w[0] = c2(g0 * v[0].x + g3 * v[1].x + g4 * v[2].x, g0 * v[0].y + g3 * v[1].y + g4 * v[2].y);
+      w[1] = c2(g3 * v[0].x + g1 * v[1].x + g5 * v[2].x, g3 * v[0].y + g1 * v[1].y + g5 * v[2].y);
+      w[2] = c2(g4 * v[0].x + g5 * v[1].x + g2 * v[2].x, g4 * v[0].y + g5 * v[1].y + g2 * v[2].y);
+      const double wgt = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
+      double t = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) t += double(v[c].x) * double(w[c].x) + double(v[c].y) * double(w[c].y);
+      rz = fma(wgt, t, rz);
+      if (shat_d) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          sw[c] = fma(wgt, double(sh[i][c].x) * double(w[c].x) + double(sh[i][c].y) * double(w[c].y), sw[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cols[(c * N + zz) * KC + kcol] = w[c];
     }
-    const R g0 = g[k][0], g1 = g[k][1], g2 = g[k][2], g3 = g[k][3], g4 = g[k][4], g5 = g[k][5];
-    X w[3];
-    w[0] = c2(g0 * v[0].x + g3 * v[1].x + g4 * v[2].x, g0 * v[0].y + g3 * v[1].y + g4 * v[2].y);
-    w[1] = c2(g3 * v[0].x + g1 * v[1].x + g5 * v[2].x, g3 * v[0].y + g1 * v[1].y + g5 * v[2].y);
-    w[2] = c2(g4 * v[0].x + g5 * v[1].x + g2 * v[2].x, g4 * v[0].y + g5 * v[1].y + g2 * v[2].y);
-    const double wgt = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
-    double t = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) t += double(v[c].x) * double(w[c].x) + double(v[c].y) * double(w[c].y);
-    rz = fma(wgt, t, rz);
-    if (shat_d) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        sw[c] = fma(wgt, double(sh[k][c].x) * double(w[c].x) + double(sh[k][c].y) * double(w[c].y), sw[c]);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) cols[(c * N + zz) * KC + kcol] = w[c];
   }
   __syncthreads();
-  sfft<true, N, typename C::ZCols, T>(cols, tw, 1);
+  sfft<true, N, typename Z::ZCols, T>(cols, tw, 1);
   for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
     const int kcol = e % KC, cz = e / KC;
-    S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
+    if (Z::KFULL || kcol < kcn) S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
   }
   double t[4] = {rz, sw[0], sw[1], sw[2]};
   block_sums<4>(t, red);
   if (threadIdx.x == 0) {
-    double* pg = A.pg + (int64_t(l) * C::NKB * N + blk) * 4;
+    double* pg = A.pg + (int64_t(l) * Z::NKB * N + blk) * 4;
     pg[0] = t[0];
     pg[1] = t[1];
     pg[2] = t[2];
@@ -411,7 +444,8 @@ void fxyi_static(FFTArgsT<R> A) {
   StageIn<N, T, X> twl;
   in.load(A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX, [](int e) { return e; });
   twl.load(A.twy, [](int e) { return e; });
-  const double* pgl = A.pg + int64_t(l) * C::NKB * N * 4;
+  constexpr int NKB = ZPass<N, R>::NKB;
+  const double* pgl = A.pg + int64_t(l) * NKB * N * 4;
   // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
   const bool update = A.p != nullptr && !A.st.first[l];
   double mean = 0.0, beta = 0.0;
@@ -419,7 +453,7 @@ void fxyi_static(FFTArgsT<R> A) {
     // one pass: r.z (offset 0) and this component's s.w (offset 1 + c) partials
     const int off[2] = {0, 1 + c};
     double sums[2];
-    wave_reduce_partials_n<2>(pgl, off, C::NKB * N, 4, sums);
+    wave_reduce_partials_n<2>(pgl, off, NKB * N, 4, sums);
     if (A.shat) mean = sums[1] / double(NN);
     if (update) {
       const double rz = A.st.rz[l];
@@ -543,13 +577,27 @@ int fxy_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
 template <int N, typename R>
 int fz_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   using C = Cube<N>;
-  const size_t lds = C::lds_z(sizeof(cplx<R>));
-  dim3 grid(N, C::NKB, L), block(C::TZ);
+  using Z = ZPass<N, R>;
+  const size_t lds = Z::lds;
+  // the per-block partials are laid out for the geometry's chunk count
+  PGJ_REQUIRE(Z::NKB <= A.f.nkb, "z pass chunks (%d) exceed the partial layout (%d)", Z::NKB, A.f.nkb);
+  dim3 grid(N, Z::NKB, L), block(Z::TZ);
   switch (mode) {
-    case kFzScale: hipLaunchKernelGGL((fz_static<N, kFzScale, R>), grid, block, lds, s, A); break;
-    case kFzPre: hipLaunchKernelGGL((fz_static<N, kFzPre, R>), grid, block, lds, s, A); break;
-    case kFzInit: hipLaunchKernelGGL((fz_static<N, kFzInit, R>), grid, block, lds, s, A); break;
-    default: hipLaunchKernelGGL((fz_static<N, kFzPCG, R>), grid, block, lds, s, A);
+    case kFzScale:
+      if (int e = allow_lds_static(fz_static<N, kFzScale, R>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzScale, R>), grid, block, lds, s, A);
+      break;
+    case kFzPre:
+      if (int e = allow_lds_static(fz_static<N, kFzPre, R>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzPre, R>), grid, block, lds, s, A);
+      break;
+    case kFzInit:
+      if (int e = allow_lds_static(fz_static<N, kFzInit, R>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzInit, R>), grid, block, lds, s, A);
+      break;
+    default:
+      if (int e = allow_lds_static(fz_static<N, kFzPCG, R>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzPCG, R>), grid, block, lds, s, A);
   }
   PGJ_LAUNCHED("fz_static");
   return PGJ_OK;

@@ -181,7 +181,8 @@ inline FFTGeom fft_geom(const pgj_grid& g) {
   int kc = 1;
   while (kc < f.nkx && size_t(kc + 1) * col + 64 * 8 <= 56 * 1024) ++kc;
   if (g.nx == g.ny && g.ny == g.nz) {
-    // same kx chunking as the compile-time kernels (fft_static.hip, Cube<N>::KC)
+    // cubes: at least as many chunks as the compile-time kernels use (fft_static.hip, ZPass<N, R>::NKB), so the
+    // per-block partials laid out with nkb below hold either kind of z pass
     switch (g.nx) {
       case 16: kc = 9; break;
       case 24: kc = 13; break;

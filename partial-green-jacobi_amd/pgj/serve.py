@@ -285,6 +285,9 @@ class Server:
         heapq.heapify(heap)
         done = []
         self.unserved = 0
+        # wall seconds per phase of the loop: starts, chunk launches, the status read (waits for the chunks),
+        # completions (certification, C^H, energies, OC update)
+        self.phases = dict(start=0.0, advance=0.0, status=0.0, complete=0.0, ticks=0)
         while heap or pending or any(self.solving[n] for n in self.sizes):
             t = now()
             while heap and heap[0][0] <= t and (horizon is None or heap[0][0] <= horizon):
@@ -297,6 +300,7 @@ class Server:
             if deadline is not None and t > deadline:
                 self.unserved = len(pending) + len(heap)
                 pending, heap = [], []
+            t_start = time.perf_counter()
             for req in pending:
                 c = req.client
                 if c.busy or c.done >= c.steps or not self._admit(c):
@@ -305,6 +309,7 @@ class Server:
                 req.start = now()
                 self._start(req)
                 started.append(req)
+            self.phases['start'] += time.perf_counter() - t_start
             pending = [r for r in pending if r not in started and r.client.done < r.client.steps]
             if not any(self.solving[n] for n in self.sizes):
                 if heap:
@@ -316,11 +321,20 @@ class Server:
                 # clients hold (their remaining steps never arrived)
                 self.unserved += len(pending)
                 break
+            t1 = time.perf_counter()
             self._advance()
-            for n, slot in self._finished():
+            t2 = time.perf_counter()
+            finished = self._finished()
+            t3 = time.perf_counter()
+            for n, slot in finished:
                 req = self.solving[n][slot]
                 if self._complete(n, slot, now()):
                     done.append(req)
+            t4 = time.perf_counter()
+            self.phases['advance'] += t2 - t1
+            self.phases['status'] += t3 - t2
+            self.phases['complete'] += t4 - t3
+            self.phases['ticks'] += 1
         torch.cuda.synchronize()
         self.elapsed = now()
         return done
