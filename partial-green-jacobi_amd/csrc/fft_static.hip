@@ -145,7 +145,8 @@ void fxy_static(FFTArgsT<R> A) {
     if (MODE == kFxyPCG && i0 == 0) {
       double pq, rz;
       constexpr int NKB = ZPass<N, R>::NKB;
-      wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * NKB * N * 4,
+      // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
+      wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * A.f.nkb * N * 4,
                              NKB * N, 4, pq, rz);
       const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
       alpha = valid ? rz / pq : 0.0;
@@ -398,7 +399,8 @@ void fz_static(FFTArgsT<R> A) {
   double t[4] = {rz, sw[0], sw[1], sw[2]};
   block_sums<4>(t, red);
   if (threadIdx.x == 0) {
-    double* pg = A.pg + (int64_t(l) * Z::NKB * N + blk) * 4;
+    // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
+    double* pg = A.pg + (int64_t(l) * A.f.nkb * N + blk) * 4;
     pg[0] = t[0];
     pg[1] = t[1];
     pg[2] = t[2];
@@ -445,7 +447,8 @@ void fxyi_static(FFTArgsT<R> A) {
   in.load(A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX, [](int e) { return e; });
   twl.load(A.twy, [](int e) { return e; });
   constexpr int NKB = ZPass<N, R>::NKB;
-  const double* pgl = A.pg + int64_t(l) * NKB * N * 4;
+  // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
+  const double* pgl = A.pg + int64_t(l) * A.f.nkb * N * 4;
   // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
   const bool update = A.p != nullptr && !A.st.first[l];
   double mean = 0.0, beta = 0.0;

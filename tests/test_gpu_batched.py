@@ -83,3 +83,65 @@ def test_warm_start_second_solve_is_cheaper():
     second, _ = bd.solve(rtol=1e-8)
     assert all(i['iterations'] == 0 for i in second[16])
     assert sum(i['iterations'] for i in first[16]) > 0
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_design_slices_at_96(precision):
+    """Designs started one at a time (slice launches) and advanced together (whole-set launches) at 96^3, where
+    the z pass uses fewer kx chunks than the per-load partials are laid out for: each design matches its solve
+    alone (C^H within 1e-10, every load certified)."""
+    from pgj.batched import DesignGroup
+    from pgj.mixed import MixedResident
+    n = 96
+    print(f'design slices n={n} precision={precision}', flush=True)
+    fields = [design.simp(design.filtered_noise(n, 20261104 + k, 2), 1e-4) for k in (1, 2)]
+    if precision == 'fp64':
+        both = DesignGroup(n, 2)
+        for d, a in enumerate(fields):
+            both.material(d, torch.tensor(a, device='cuda'))
+            both.select(d, .5)
+        both.u.zero_()
+        both.setup(1e-8)
+        both.certify()
+        for d in range(2):           # each design armed by its own slice launch
+            both.initialize(d, 1)
+        for _ in range(400):
+            both.iteration()
+            if not bool(both.active.any()):
+                break
+        both.certify()
+        assert not bool(both.active.any()) and not bool(both.bad.any())
+        for d, a in enumerate(fields):
+            solo = pgj.Resident(n, load_mode='parallel', batch=6).capture()
+            solo.material(torch.tensor(a, device='cuda'))
+            solo.select(.5)
+            sinfo, _ = solo.solve(rtol=1e-8)
+            assert all(i['converged'] for i in sinfo)
+            assert rel(both.effective_tensor(d), solo.effective_tensor().cpu().numpy()) < 1e-9, d
+    else:
+        both = MixedResident(n, designs=2)
+        for d, a in enumerate(fields):
+            both.material(d, torch.tensor(a, device='cuda'))
+            both.select(d, .5)
+        both.setup(1e-8)
+        both.certify()
+        for _ in range(30):          # refinement steps: begin / end / certify by slice, inner iterations together
+            v = both.status()
+            if (np.isfinite(v[:, 1]) & (v[:, 1] <= v[:, 2])).all():
+                break
+            for d in range(2):
+                both.begin(d, 1)
+            while bool(both.active32.any()):
+                both.iteration()
+            assert not bool(both.bad32.any())
+            for d in range(2):
+                both.end(d, 1)
+                both.certify(d, 1)
+        v = both.status()
+        assert (v[:, 1] <= v[:, 2]).all()
+        for d, a in enumerate(fields):
+            alone = MixedResident(n)
+            alone.material(0, torch.tensor(a, device='cuda'))
+            alone.select(0, .5)
+            alone.solve(rtol=1e-8)
+            assert rel(both.effective_tensor(d), alone.effective_tensor(0).cpu().numpy()) < 1e-10, d
