@@ -43,19 +43,21 @@ struct Cube {
 
 // z pass of a cube in precision R: KC kx columns of all three components per
 // block; a row of KC complex values is the contiguous run a block reads per
-// (c, kz). The last chunk is partial when KC does not divide NKX (96: 17 + 17
-// + 15). 128^3 FP64 keeps 5 columns: 13 would need 147 VGPRs at 2 blocks per CU.
+// (c, kz). A last chunk narrower than KC is supported (columns past NKX are
+// zero in LDS and skipped). 128^3 takes 13 columns in FP32 (104-byte runs
+// instead of 40) but keeps 5 in FP64, where 13 would need 147 VGPRs at 2 blocks
+// per CU; 17 columns at 96^3 measured no faster than 7.
 template <int N, typename R>
 struct ZPass {
   static constexpr int NKX = N / 2 + 1;
-  static constexpr int TZ = N >= 96 ? 512 : 256;
   static constexpr int KC = N == 16    ? 9
                             : N == 24  ? 13
                             : N == 32  ? 17
                             : N == 48  ? 5
                             : N == 64  ? 11
-                            : N == 96  ? 17
+                            : N == 96  ? 7
                             : sizeof(R) == 8 ? 5 : 13;
+  static constexpr int TZ = KC >= 13 && N >= 96 ? 512 : 256;
   static constexpr int NKB = (NKX + KC - 1) / KC;
   static constexpr bool KFULL = NKX % KC == 0;
   // columns and the twiddle table; the reduction reuses the column storage once

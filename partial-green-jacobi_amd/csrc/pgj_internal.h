@@ -90,10 +90,12 @@ __device__ __forceinline__ void block_sums(double (&v)[K], double* scratch) {
 
 // Deterministic reduction of `count` partials at `p` (stride `stride`) by one
 // wavefront: lane-strided accumulation followed by wave_sum. Every lane of
-// every calling wavefront obtains bit-identical results.
+// every calling wavefront obtains bit-identical results. The loop is unrolled
+// so that a lane's loads are in flight together; the additions keep their order.
 __device__ __forceinline__ double wave_reduce_partials(const double* p, int count, int stride) {
   const int lane = threadIdx.x & 63;
   double v = 0.0;
+#pragma unroll 8
   for (int i = lane; i < count; i += 64) v += p[(int64_t)i * stride];
   return wave_sum(v);
 }
@@ -109,6 +111,7 @@ __device__ __forceinline__ void wave_reduce_partials_n(const double* p, const in
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = 0.0;
+#pragma unroll 8
   for (int i = lane; i < count; i += 64) {
     const double* q = p + (int64_t)i * stride;
 #pragma unroll
@@ -125,6 +128,7 @@ __device__ __forceinline__ void wave_reduce_partials_2(const double* p, int pc, 
   const int lane = threadIdx.x & 63;
   double va = 0.0, vb = 0.0;
   const int n = pc > qc ? pc : qc;
+#pragma unroll 8
   for (int i = lane; i < n; i += 64) {
     if (i < pc) va += p[(int64_t)i * ps];
     if (i < qc) vb += q[(int64_t)i * qs];
