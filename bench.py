@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each stage at 64^3 with 6 loads, from the FETCH_SIZE / WRITE_SIZE counter runs of
 # `bench.py --profile-stages` scaled by a 256 MiB copy (read factor 2.0, write factor 1.0);
 # profiles/pcg_stages_64cube_6loads_traffic.md holds the commands and the table.
-MEASURED_TRAFFIC_64_6 = {'hex8_rowmarch': 93.7e6, 'fxy': 162.8e6, 'fz': 122.6e6, 'fxyi': 207.5e6}
+MEASURED_TRAFFIC_64_6 = {'hex8_rowmarch': 93.7e6, 'fxy': 162.7e6, 'fz': 121.6e6, 'fxyi': 208.6e6}
 
 
 def parse():
