@@ -21,40 +21,18 @@ namespace pgj {
 // bounds layout (cone):     [lo, hi, running (1/0), bisection count]
 // bounds layout (gaussian): [lo, hi]
 
-// Clipped cone filter y_i = sum_j w_j x[i + o_j] over in-range neighbours, in
-// offset order. One 8x8x8 output tile per block: the tile and a halo of
-// kConeHalo voxels go to LDS once (zeros outside the grid), and every offset
-// within the halo reads LDS; offsets beyond it (cone radius above 4) read
-// global memory. The offsets are uniform across the block.
-constexpr int kConeTile = 8, kConeHalo = 3, kConeSpan = kConeTile + 2 * kConeHalo;
-__global__ __launch_bounds__(512) void cone_filter_tiled(const double* x, const int* off, const double* w, int m, int n,
-                                                         double* y, const double* bounds, int conditional) {
-  __shared__ double tile[kConeSpan * kConeSpan * kConeSpan];
-  if (conditional && bounds[2] == 0.0) return;
-  const int bx = blockIdx.x * kConeTile, by = blockIdx.y * kConeTile, bz = blockIdx.z * kConeTile;
-  for (int e = threadIdx.x; e < kConeSpan * kConeSpan * kConeSpan; e += blockDim.x) {
-    const int tx = e % kConeSpan, ty = (e / kConeSpan) % kConeSpan, tz = e / (kConeSpan * kConeSpan);
-    const int gx = bx + tx - kConeHalo, gy = by + ty - kConeHalo, gz = bz + tz - kConeHalo;
-    const bool in = gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n;
-    tile[e] = in ? x[(int64_t(gz) * n + gy) * n + gx] : 0.0;
-  }
-  __syncthreads();
-  const int lx = threadIdx.x % kConeTile, ly = (threadIdx.x / kConeTile) % kConeTile, lz = threadIdx.x / (kConeTile * kConeTile);
-  const int xx = bx + lx, yy = by + ly, zz = bz + lz;
-  if (xx >= n || yy >= n || zz >= n) return;
+__global__ __launch_bounds__(256) void cone_filter_kernel(const double* x, const int* off, const double* w, int m, int n,
+                                                          double* y, const double* bounds, int conditional) {
+  const int64_t N = int64_t(n) * n * n;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N || (conditional && bounds[2] == 0.0)) return;
+  const int xx = int(i % n), yy = int((i / n) % n), zz = int(i / (int64_t(n) * n));
   double sum = 0.0;
   for (int j = 0; j < m; ++j) {
-    const int ox = off[3 * j], oy = off[3 * j + 1], oz = off[3 * j + 2];
-    if (abs(ox) <= kConeHalo && abs(oy) <= kConeHalo && abs(oz) <= kConeHalo) {
-      const int xj = xx + ox, yj = yy + oy, zj = zz + oz;
-      if (xj >= 0 && xj < n && yj >= 0 && yj < n && zj >= 0 && zj < n)
-        sum += w[j] * tile[((lz + kConeHalo + oz) * kConeSpan + ly + kConeHalo + oy) * kConeSpan + lx + kConeHalo + ox];
-    } else {
-      const int xj = xx + ox, yj = yy + oy, zj = zz + oz;
-      if (xj >= 0 && xj < n && yj >= 0 && yj < n && zj >= 0 && zj < n) sum += w[j] * x[(int64_t(zj) * n + yj) * n + xj];
-    }
+    const int xj = xx + off[3 * j], yj = yy + off[3 * j + 1], zj = zz + off[3 * j + 2];
+    if (xj >= 0 && xj < n && yj >= 0 && yj < n && zj >= 0 && zj < n) sum += w[j] * x[(int64_t(zj) * n + yj) * n + xj];
   }
-  y[(int64_t(zz) * n + yy) * n + xx] = sum;
+  y[i] = sum;
 }
 
 // eta-projection: rho = (tanh(b eta) + tanh(b (t/hs - eta))) / den, with volume partials.
@@ -268,9 +246,9 @@ extern "C" int pgj_cone_filter(int n, const double* x, const int32_t* offsets, c
                                double* y, const double* bounds, int conditional, void* stream) {
   PGJ_REQUIRE(n >= 2 && x && offsets && weights && y && m >= 1, "invalid cone filter arguments");
   PGJ_REQUIRE(!conditional || bounds, "conditional filtering needs bounds");
-  const unsigned tiles = unsigned((n + kConeTile - 1) / kConeTile);
-  hipLaunchKernelGGL(cone_filter_tiled, dim3(tiles, tiles, tiles), dim3(512), 0, as_stream(stream), x, offsets, weights,
-                     m, n, y, bounds, conditional);
+  const int64_t N = int64_t(n) * n * n;
+  hipLaunchKernelGGL(cone_filter_kernel, dim3(blocks_of(N)), dim3(256), 0, as_stream(stream), x, offsets, weights, m,
+                     n, y, bounds, conditional);
   PGJ_LAUNCHED("cone_filter");
   return PGJ_OK;
 }
