@@ -160,6 +160,28 @@ size_t pgj_resident_ws_bytes(const pgj_grid* g, int L);
  * complex128. Replaces Resident.select (resident.py:156-174). */
 int pgj_resident_scale(const pgj_grid* g, const void* plan, const double* a, double alpha, double* s, void* s_hat,
                        void* ws, void* stream);
+/* pgj_resident_scale with alpha read from device memory (one double), as
+ * written by pgj_alpha_select; selection and scaling then run without a host
+ * round trip, as the reference's torch ops do (resident.py:156-174). */
+int pgj_resident_scale_dev(const pgj_grid* g, const void* plan, const double* a, const double* alpha, double* s,
+                           void* s_hat, void* ws, void* stream);
+
+/* ------------------------------------------------------- alpha selectors */
+/* On-device choice of alpha for the field a (N) and, for the gray selector,
+ * the physical density rho (N). method: 0 fixed (params[0] = alpha; the eta
+ * selector is this with the anchor of the Emin decade chosen on the host,
+ * run.py:52), 1 green (0), 2 gj (1), 4 contrast (params = anchors b4, b6, b9;
+ * run.py:28-33), 5 gray (params = b4, b6, b9, gamma, g_ref; run.py:37-43),
+ * 6 compact8 and 7 compact8q (law features of a in FP32,
+ * solver/experiments/global_alpha_compact_law/law.py:9-18, and the formula of
+ * resident.py:163-168 / run.py:60-66). alpha (1 double, device) receives the
+ * value; features (nullable, 7 doubles, device) receive log10 contrast
+ * clamped to [0, 9], <rho>, <rho^2>, area, jump, law contrast, complexity.
+ * ws holds pgj_alpha_ws_bytes(g) bytes. Replaces Resident.select /
+ * SelectorResident.choose (resident.py:156-168, run.py:53-67). */
+size_t pgj_alpha_ws_bytes(const pgj_grid* g);
+int pgj_alpha_select(const pgj_grid* g, int method, const double* params, const double* a, const double* rho,
+                     double* ws, double* features, double* alpha, void* stream);
 /* Resident sets: the L loads of a resident set form L/group consecutive
  * designs of `group` loads each (group divides L, L <= 4096). Design d owns
  * a[d] (N), s[d] (3, N) and s_hat[d] (3, nz, ny, nx/2+1); group = L is one

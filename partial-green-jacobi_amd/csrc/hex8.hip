@@ -705,8 +705,10 @@ __global__ __launch_bounds__(256) void diagonal_kernel(int nz, int ny, int nx, c
 }
 
 __global__ __launch_bounds__(256) void scale_kernel(int nz, int ny, int nx, const double* a, DiagTable ed,
-                                                    double alpha, const double* logpart, int nblk, double* s,
-                                                    int soa) {
+                                                    double alpha_value, const double* alpha_dev, const double* logpart,
+                                                    int nblk, double* s, int soa) {
+  // alpha from device memory (an on-device selection) when given
+  const double alpha = alpha_dev ? *alpha_dev : alpha_value;
   __shared__ double smean;
   const int64_t N = int64_t(nx) * ny * nz;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -872,13 +874,14 @@ int launch_rhs(const pgj_grid& g, const double* a, const double* macro, double* 
 }
 
 int launch_scale(const pgj_grid& g, const double* a, double alpha, double* d, double* s_out, double* ws, int soa,
-                 hipStream_t s) {
+                 hipStream_t s, const double* alpha_dev) {
   const int64_t N = int64_t(g.nx) * g.ny * g.nz;
   const int nblk = int((N + 255) / 256);
   DiagTable t = diag_table(g);
   hipLaunchKernelGGL(diagonal_kernel, dim3(nblk), dim3(256), 0, s, g.nz, g.ny, g.nx, a, t, d, ws);
   PGJ_LAUNCHED("diagonal_kernel");
-  hipLaunchKernelGGL(scale_kernel, dim3(nblk), dim3(256), 0, s, g.nz, g.ny, g.nx, a, t, alpha, ws, nblk, s_out, soa);
+  hipLaunchKernelGGL(scale_kernel, dim3(nblk), dim3(256), 0, s, g.nz, g.ny, g.nx, a, t, alpha, alpha_dev, ws, nblk,
+                     s_out, soa);
   PGJ_LAUNCHED("scale_kernel");
   return PGJ_OK;
 }

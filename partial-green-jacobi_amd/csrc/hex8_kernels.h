@@ -54,7 +54,8 @@ int launch_reference(const pgj_grid& g, int L, RefArgs& A, bool cert, hipStream_
 int launch_component_sums(const pgj_grid& g, int L, const double* v, double* part, int soa, hipStream_t s);
 int launch_subtract_mean(const pgj_grid& g, int L, double* v, const double* part, int soa, hipStream_t s);
 int launch_rhs(const pgj_grid& g, const double* a, const double* macro, double* out, int soa, hipStream_t s);
+// alpha_dev (nullable): read alpha from device memory instead of `alpha`
 int launch_scale(const pgj_grid& g, const double* a, double alpha, double* d, double* s_out, double* ws, int soa,
-                 hipStream_t s);
+                 hipStream_t s, const double* alpha_dev = nullptr);
 
 }  // namespace pgj

@@ -259,6 +259,19 @@ extern "C" int pgj_resident_scale(const pgj_grid* g, const void* plan, const dou
   return fz(A, *g, 1, kFzScale, s);
 }
 
+extern "C" int pgj_resident_scale_dev(const pgj_grid* g, const void* plan, const double* a, const double* alpha,
+                                      double* sc, void* s_hat, void* ws, void* stream) {
+  if (int e = check_fft(g)) return e;
+  PGJ_REQUIRE(plan && a && alpha && sc && s_hat && ws, "invalid resident scale arguments");
+  hipStream_t s = as_stream(stream);
+  ResidentBuffers b = resident_buffers<double>(*g, 1, ws);
+  if (int e = launch_scale(*g, a, 0.0, nullptr, sc, b.pcert, 1, s, alpha)) return e;
+  FFTArgs A = fft_args(*g, plan);
+  A.s = sc; A.spec = static_cast<double2*>(s_hat); A.pr = b.pr; A.pg = b.pg; A.soa = 1;
+  if (int e = fxy(A, *g, 1, kFxyScale, s)) return e;
+  return fz(A, *g, 1, kFzScale, s);
+}
+
 extern "C" int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan,
                                  const double* sc, const void* s_hat, const double* r, double* p, void* state,
                                  void* ws, void* stream) {

@@ -4,9 +4,10 @@ Host-side mirror of the reference's operator / preconditioner / PCG / resident
 interfaces over the HIP C ABI in ``libpgj.so`` (declared in ``include/pgj.h``).
 """
 from . import _lib, pcg
+from .driver import compute_effective_stiffness_batched
 from .operator import Operator, build, diagonal, scale, set_alpha
 from .pcg import certified_solve, solve_pcg_batched
 from .resident import Resident
 
-__all__ = ['Operator', 'Resident', 'build', 'certified_solve', 'diagonal', 'scale', 'set_alpha',
-           'solve_pcg_batched']
+__all__ = ['Operator', 'Resident', 'build', 'certified_solve', 'compute_effective_stiffness_batched', 'diagonal',
+           'scale', 'set_alpha', 'solve_pcg_batched']

@@ -70,6 +70,7 @@ class Resident:
         self.scale_hat = torch.zeros((3, nz * ny * (nx // 2 + 1)), dtype=torch.complex128, device=dev)
         self._set_scale(0.0)
         self.alpha = torch.zeros((), dtype=torch.float64, device=dev)
+        self._selector = None
         self.energy_out = z(nl, N)
         self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
         self.ch_out = z(36)
@@ -158,8 +159,21 @@ class Resident:
         self.op._material_field.copy_(torch.as_tensor(a, dtype=torch.float64).reshape(self.shape))
         self.op._diagonal = None
 
-    def select(self, method):
-        """alpha by name ('green', 'gj') or value; rebuilds s and its spectrum (``resident.py:156-174``)."""
+    def select(self, method, emin=None, rho=None):
+        """alpha by value, 'green' or 'gj', or by a density selector ('eta', 'contrast', 'gray', 'compact8',
+        'compact8q') evaluated on the device; rebuilds s and its spectrum (``resident.py:156-174``,
+        ``exp/asg_density_selector_20261105/run.py:53-67``). ``emin`` is needed by 'eta', the physical density
+        ``rho`` by 'gray'. Returns alpha (a float, or the device scalar for a selector)."""
+        from .select import Selector, _METHOD
+        if isinstance(method, str) and (method == 'eta' or method in _METHOD) and method not in ('green', 'gj'):
+            if self._selector is None:
+                self._selector = Selector(self.op.grid, self.op.device)
+            alpha = self._selector(self.op._material_field, method, emin, rho)
+            self.alpha.copy_(alpha)
+            _lib.call('pgj_resident_scale_dev', self.op.grid, _lib.ptr(self.op._plan),
+                      _lib.ptr(self.op._material_field), _lib.ptr(self.alpha), _lib.ptr(self.scale),
+                      _lib.ptr(self.scale_hat), _lib.ptr(self.ws), _lib.stream())
+            return self.alpha
         if method == 'green':
             alpha = 0.0
         elif method == 'gj':

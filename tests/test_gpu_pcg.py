@@ -171,3 +171,23 @@ def test_certified_solve_matches_backend(x0):
     np.testing.assert_allclose(info['rhs_norm'], oinfo['rhs_norm'], rtol=1e-12)
     true = np.linalg.norm(f - hex8.apply(a, u.cpu().numpy(), C0))
     np.testing.assert_allclose(true, info['absolute_residual'], rtol=1e-6, atol=1e-20)
+
+
+def test_effective_stiffness_batched_matches_energy_tensor():
+    """The average-stress C^H route (six drop-in PCG solves, mean stress per load) equals the energy C^H of the
+    oracle's resident solve and the oracle's own mean stresses on its solutions."""
+    n = 8
+    a = design.simp(design.filtered_noise(n, 31, 2), 1e-4)
+    op = pgj.build(torch.tensor(a, device='cuda'))
+    pgj.set_alpha(op, .5)
+    C, info = pgj.compute_effective_stiffness_batched(op, tol=1e-10, max_iter=2000)
+    assert C.shape == (1, 6, 6) and len(info) == 1 and len(info[0]) == 6
+    assert all(i['converged'] and i['rel_res'] < 1e-10 and not i['residual_verified'] for i in info[0])
+    np.testing.assert_array_equal(C[0], C[0].T)
+    ref = opcg.ResidentPCG(a, C0, np.eye(6), batch=6, s=green.scale(a, .5, C0))
+    ref.solve(rtol=1e-12)
+    H = design.effective_tensor(a, ref.solutions, C0)
+    assert np.abs(C[0] - H).max() / np.abs(H).max() < 1e-9
+    S = np.stack([hex8.avg_stress(a, ref.solutions[j:j + 1], np.eye(6)[j], C0) for j in range(6)], axis=1)
+    S = 0.5 * (S + S.T)
+    assert np.abs(C[0] - S).max() / np.abs(S).max() < 1e-9
