@@ -176,7 +176,8 @@ int pgj_resident_scale_dev(const pgj_grid* g, const void* plan, const double* a,
  * solver/experiments/global_alpha_compact_law/law.py:9-18, and the formula of
  * resident.py:163-168 / run.py:60-66). alpha (1 double, device) receives the
  * value; features (nullable, 7 doubles, device) receive log10 contrast
- * clamped to [0, 9], <rho>, <rho^2>, area, jump, law contrast, complexity.
+ * clamped to [0, 9], <rho>, <rho^2> (0 without rho) and area, jump, law
+ * contrast, complexity (0 unless the method is 6 or 7).
  * ws holds pgj_alpha_ws_bytes(g) bytes. Replaces Resident.select /
  * SelectorResident.choose (resident.py:156-168, run.py:53-67). */
 size_t pgj_alpha_ws_bytes(const pgj_grid* g);
