@@ -349,6 +349,31 @@ int pgj_pcg_pre_step(int B, int64_t n, double* u, double* r, const double* p, co
 int pgj_pcg_post_step(int B, int64_t n, const double* r, double* z, const double* minv, int64_t minv_stride,
                       double* p, double* rz, double* ws, void* stream);
 
+/* ---- general-C21 material (hybrid storage) ----
+ * Element e has the stiffness C_e given by its 21 upper-triangle Voigt
+ * entries in row-major order ((0,0),(0,1),...,(0,5),(1,1),...,(5,5), as
+ * solver/src/materials/material_field.py:14 PAIRS), read from
+ * pool + 21*index[e] (index != NULL: shared phase rows and per-voxel rows,
+ * solver/src/materials/hybrid_material.py:14-70) or pool + 21*e (index NULL,
+ * the dense ablation). Only the grid's extents and voxel sizes are used. */
+
+/* out_l = K(C) u_l for L AoS (L, 3N) loads; active (uint8[L], nullable)
+ * zeroes inactive loads. A per-node gather over the eight elements in a
+ * fixed order (no atomics). Replaces `apply`
+ * (solver/src/materials/hybrid_kernels.cu:25-48, host :117-127). */
+int pgj_hybrid_apply(const pgj_grid* g, int L, const double* pool, const int32_t* index, const double* u,
+                     const uint8_t* active, double* out, void* stream);
+/* f = -V sum_e Bbar_e^T C_e eps(macro), (1, 3N); macro is 6 host doubles.
+ * Replaces `rhs` (solver/src/materials/hybrid_kernels.cu:49-68, host :128-137). */
+int pgj_hybrid_rhs(const pgj_grid* g, const double* pool, const int32_t* index, const double macro[6], double* out,
+                   void* stream);
+/* Mean stress <C_e (eps(macro) + Bbar u_e)> (6 doubles on the device).
+ * Replaces `stress` (solver/src/materials/hybrid_kernels.cu:69-106, host
+ * :138-147); ws holds 6 partial sums per 256 elements. */
+size_t pgj_hybrid_stress_ws_bytes(const pgj_grid* g);
+int pgj_hybrid_avg_stress(const pgj_grid* g, const double* pool, const int32_t* index, const double* u,
+                          const double macro[6], double* ws, double* out6, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,6 +144,98 @@ PGJ_HD inline void hex8_element(const T (&u)[24], T a, const ElementCoef& c, T (
     for (int d = 0; d < 3; ++d) f[3 * n + d] = F[d][n];
 }
 
+// General anisotropic material: the element constant C (6x6 Voigt matrix,
+// order 11, 22, 33, 23, 13, 12, engineering shear) in place of a C0. The same
+// monomial decomposition holds for any constant C; the stress of strain
+// monomial t is k_t C eps_t with eps_t = (G00, G11, G22, G12 + G21, G02 + G20,
+// G01 + G10), and the modal force of component d along x_j is g_j sigma_dj.
+// c.g and c.vol give the voxel (c.lam, c.mu are not used).
+template <typename T = double>
+PGJ_HD inline void hex8_modal_core_c(const T (&U)[3][8], const T (&C)[6][6], const ElementCoef& c, T (&F)[3][8]) {
+  // tensor index (d, j) -> Voigt row
+  constexpr int kVoigt[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) F[d][n] = T(0);
+#pragma unroll
+  for (int t = 0; t < 7; ++t) {
+    const int bits = ((t & 1) ? 1 : 0) + ((t & 2) ? 1 : 0) + ((t & 4) ? 1 : 0);
+    const T k = T(c.vol) * (bits == 0 ? T(1) : (bits == 1 ? T(1.0 / 3.0) : T(1.0 / 9.0)));
+    T G[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G[d][j] = (t & (1 << j)) ? T(0) : T(c.g[j]) * U[d][t | (1 << j)];
+    const T eps[6] = {G[0][0], G[1][1], G[2][2], G[1][2] + G[2][1], G[0][2] + G[2][0], G[0][1] + G[1][0]};
+    T sig[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      T acc = T(0);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc += C[r][q] * eps[q];
+      sig[r] = k * acc;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (!(t & (1 << j))) F[d][t | (1 << j)] += T(c.g[j]) * sig[kVoigt[d][j]];
+  }
+}
+
+// f = Ke(C) u for one element (dof 3a+d), C as in hex8_modal_core_c.
+template <typename T = double>
+PGJ_HD inline void hex8_element_c(const T (&u)[24], const T (&C)[6][6], const ElementCoef& c, T (&f)[24]) {
+  T U[3][8];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) U[d][n] = u[3 * n + d];
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        if (!(n & b)) {
+          const T lo = U[d][n], hi = U[d][n | b];
+          U[d][n] = lo + hi;
+          U[d][n | b] = hi - lo;
+        }
+  T F[3][8];
+  hex8_modal_core_c<T>(U, C, c, F);
+#pragma unroll
+  for (int b = 4; b >= 1; b >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        if (!(n & b)) {
+          const T lo = F[d][n], hi = F[d][n | b];
+          F[d][n] = lo - hi;
+          F[d][n | b] = lo + hi;
+        }
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[3 * n + d] = F[d][n];
+}
+
+// Unpack a 21-entry upper triangle (rows of the 6x6 Voigt matrix: C00..C05,
+// C11..C15, C22..C25, C33..C35, C44, C45, C55) into the full symmetric matrix.
+template <typename T = double>
+PGJ_HD inline void unpack_c21(const double* p, T (&C)[6][6]) {
+  int k = 0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int q = r; q < 6; ++q) {
+      C[r][q] = C[q][r] = T(p[k]);
+      ++k;
+    }
+}
+
 inline ElementCoef make_coef(double hx, double hy, double hz, double lam, double mu) {
   ElementCoef c;
   c.g[0] = 1.0 / (4.0 * hx);

@@ -1,0 +1,188 @@
+// General-C21 material (the reference's hybrid operator): element e carries a
+// constant anisotropic C, the 21-entry upper triangle of its 6x6 Voigt matrix,
+// either as row e of a coefficient array (direct) or as row index[e] of a
+// shared pool (indirect: phases share one row, multi-phase composites). The
+// apply gathers per node over its eight elements (fixed order, no atomics, so
+// results are independent of the launch); RHS and mean stress use the exact
+// centroid gradient of the trilinear field.
+#include "hex8_element.h"
+#include "hex8_kernels.h"
+#include "layout.h"
+#include "pgj_internal.h"
+
+namespace pgj {
+
+// One Voigt macro strain (engineering shears), passed by value.
+struct Strain6 {
+  double v[6];
+};
+
+__device__ __forceinline__ const double* c21_row(const double* pool, const int32_t* index, int64_t e) {
+  return pool + (index ? int64_t(index[e]) : e) * 21;
+}
+
+// q_l = sum_e P_e^T Ke(C_e) P_e u_l; AoS (L, 3N); inactive loads give zeros.
+__global__ __launch_bounds__(256) void c21_apply_kernel(int nz, int ny, int nx, ElementCoef coef, const double* pool,
+                                                        const int32_t* index, const double* u, const uint8_t* active,
+                                                        double* out) {
+  const int l = blockIdx.y;
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t base = int64_t(l) * 3 * N;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (!active || active[l]) {
+    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
+#pragma unroll 1
+    for (int o = 0; o < 8; ++o) {
+      const int ex = wrapi(x - (o & 1), nx), ey = wrapi(y - ((o >> 1) & 1), ny), ez = wrapi(z - ((o >> 2) & 1), nz);
+      const int64_t e = (int64_t(ez) * ny + ey) * nx + ex;
+      double ue[24], fe[24], C[6][6];
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        const int64_t node = (int64_t(wrapi(ez + ((n >> 2) & 1), nz)) * ny + wrapi(ey + ((n >> 1) & 1), ny)) * nx +
+                             wrapi(ex + (n & 1), nx);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[base + 3 * node + d];
+      }
+      unpack_c21(c21_row(pool, index, e), C);
+      hex8_element_c(ue, C, coef, fe);
+      // this node is local node o of element e (static register indices)
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k == o) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) acc[d] += fe[3 * k + d];
+        }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[base + 3 * i + d] = acc[d];
+}
+
+// f = -sum_e V Bbar_e^T C_e E (the element force of the affine field of E), AoS (1, 3N).
+__global__ __launch_bounds__(256) void c21_rhs_kernel(int nz, int ny, int nx, ElementCoef coef, const double* pool,
+                                                      const int32_t* index, Strain6 E, double* out) {
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
+  double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int o = 0; o < 8; ++o) {
+    const int ex = wrapi(x - (o & 1), nx), ey = wrapi(y - ((o >> 1) & 1), ny), ez = wrapi(z - ((o >> 2) & 1), nz);
+    double C[6][6];
+    unpack_c21(c21_row(pool, index, (int64_t(ez) * ny + ey) * nx + ex), C);
+    double s[6];
+    for (int r = 0; r < 6; ++r) {
+      double acc = 0.0;
+      for (int q = 0; q < 6; ++q) acc += C[r][q] * E.v[q];
+      s[r] = coef.vol * acc;
+    }
+    const double dx = (o & 1) ? coef.g[0] : -coef.g[0];
+    const double dy = (o & 2) ? coef.g[1] : -coef.g[1];
+    const double dz = (o & 4) ? coef.g[2] : -coef.g[2];
+    f[0] -= dx * s[0] + dy * s[5] + dz * s[4];
+    f[1] -= dx * s[5] + dy * s[1] + dz * s[3];
+    f[2] -= dx * s[4] + dy * s[3] + dz * s[2];
+  }
+  for (int d = 0; d < 3; ++d) out[3 * i + d] = f[d];
+}
+
+// Per-block sums of C_e (E + Bbar u_e) for the mean stress.
+__global__ __launch_bounds__(256) void c21_stress_partials(int nz, int ny, int nx, ElementCoef coef, const double* pool,
+                                                           const int32_t* index, const double* u, Strain6 E,
+                                                           double* part) {
+  __shared__ double red[8];
+  const int64_t N = int64_t(nx) * ny * nz;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double sig[6] = {0, 0, 0, 0, 0, 0};
+  if (e < N) {
+    const int x = int(e % nx), y = int((e / nx) % ny), z = int(e / (int64_t(nx) * ny));
+    double eps[6];
+    for (int k = 0; k < 6; ++k) eps[k] = E.v[k];
+    for (int n = 0; n < 8; ++n) {
+      const int64_t node = (int64_t(wrapi(z + ((n >> 2) & 1), nz)) * ny + wrapi(y + ((n >> 1) & 1), ny)) * nx +
+                           wrapi(x + (n & 1), nx);
+      const double ux = u[3 * node], uy = u[3 * node + 1], uz = u[3 * node + 2];
+      const double dx = (n & 1) ? coef.g[0] : -coef.g[0];
+      const double dy = (n & 2) ? coef.g[1] : -coef.g[1];
+      const double dz = (n & 4) ? coef.g[2] : -coef.g[2];
+      eps[0] += dx * ux;
+      eps[1] += dy * uy;
+      eps[2] += dz * uz;
+      eps[3] += dz * uy + dy * uz;
+      eps[4] += dz * ux + dx * uz;
+      eps[5] += dy * ux + dx * uy;
+    }
+    double C[6][6];
+    unpack_c21(c21_row(pool, index, e), C);
+    for (int r = 0; r < 6; ++r) {
+      double acc = 0.0;
+      for (int q = 0; q < 6; ++q) acc += C[r][q] * eps[q];
+      sig[r] = acc;
+    }
+  }
+  for (int k = 0; k < 6; ++k) {
+    const double t = block_sum(sig[k], red);
+    if (threadIdx.x == 0) part[int64_t(blockIdx.x) * 6 + k] = t;
+  }
+}
+
+__global__ void c21_stress_finish(const double* part, int nblk, int64_t N, double* out) {
+  const int k = blockIdx.x;
+  const double s = wave_reduce_partials(part + k, nblk, 6);
+  if (threadIdx.x == 0) out[k] = s / double(N);
+}
+
+}  // namespace pgj
+
+using namespace pgj;
+
+namespace {
+int c21_blocks(const pgj_grid& g) { return int((int64_t(g.nx) * g.ny * g.nz + 255) / 256); }
+Strain6 macro_of(const double* m) {
+  Strain6 E{};
+  for (int k = 0; k < 6; ++k) E.v[k] = m[k];
+  return E;
+}
+}  // namespace
+
+extern "C" int pgj_hybrid_apply(const pgj_grid* g, int L, const double* pool, const int32_t* index, const double* u,
+                                const uint8_t* active, double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && pool && u && out, "invalid hybrid apply arguments");
+  hipLaunchKernelGGL(c21_apply_kernel, dim3(c21_blocks(*g), L), dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,
+                     coef_of(*g), pool, index, u, active, out);
+  PGJ_LAUNCHED("c21_apply_kernel");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_hybrid_rhs(const pgj_grid* g, const double* pool, const int32_t* index, const double macro[6],
+                              double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(pool && macro && out, "invalid hybrid rhs arguments");
+  hipLaunchKernelGGL(c21_rhs_kernel, dim3(c21_blocks(*g)), dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,
+                     coef_of(*g), pool, index, macro_of(macro), out);
+  PGJ_LAUNCHED("c21_rhs_kernel");
+  return PGJ_OK;
+}
+
+extern "C" size_t pgj_hybrid_stress_ws_bytes(const pgj_grid* g) {
+  if (!g) return 0;
+  return align256(size_t(c21_blocks(*g)) * 6 * 8);
+}
+
+extern "C" int pgj_hybrid_avg_stress(const pgj_grid* g, const double* pool, const int32_t* index, const double* u,
+                                     const double macro[6], double* ws, double* out6, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(pool && u && macro && ws && out6, "invalid hybrid stress arguments");
+  const int nblk = c21_blocks(*g);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(c21_stress_partials, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), pool, index, u,
+                     macro_of(macro), ws);
+  PGJ_LAUNCHED("c21_stress_partials");
+  hipLaunchKernelGGL(c21_stress_finish, dim3(6), dim3(64), 0, s, ws, nblk, int64_t(g->nx) * g->ny * g->nz, out6);
+  PGJ_LAUNCHED("c21_stress_finish");
+  return PGJ_OK;
+}

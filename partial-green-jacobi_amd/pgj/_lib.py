@@ -71,6 +71,10 @@ SIGNATURES = {
     'pgj_resident_scale_dev': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_alpha_ws_bytes': (c_size_t, [GP]),
     'pgj_alpha_select': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hybrid_apply': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hybrid_rhs': (c_int, [GP, c_void_p, c_void_p, c_dp, c_void_p, c_void_p]),
+    'pgj_hybrid_stress_ws_bytes': (c_size_t, [GP]),
+    'pgj_hybrid_avg_stress': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_dp, c_void_p, c_void_p, c_void_p]),
     'pgj_oc_ws_bytes': (c_size_t, [c_int]),
     'pgj_cone_filter': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     'pgj_cone_project': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
