@@ -47,7 +47,7 @@ def parse():
     p.add_argument('--probe-iters', type=int, default=10)
     p.add_argument('--cpu-seconds', type=float, default=15.0)
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed'), default='plain',
+    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid'), default='plain',
                    help='plain: the FP64 drop-in design step (default); batched: 48 designs of uneven grids in one '
                         'captured graph vs the same designs solved one at a time; serve: Poisson-arrival '
                         'design-step requests through the continuous-batching server; mixed: FP32-inner / '
@@ -284,6 +284,125 @@ def mixed_form(args):
     print(json.dumps(out), flush=True)
 
 
+C21_ELEMENT_FLOPS = 660   # modal C21 element: 2 x 72 butterfly flops + 516 in the seven strain modes
+
+
+def composite(n, seed, radius):
+    """Three-phase 64^3 composite: matrix (isotropic), fibre phase (local-frame orthotropic, frame turning
+    with x), stiff inclusions (isotropic); phases by quartiles of a seeded filtered-noise field."""
+    from pgj.density import filtered_noise
+    f = filtered_noise(n, seed, radius)
+    q50, q75 = np.quantile(f, [0.5, 0.75])
+    phase = np.where(f < q50, 0, np.where(f < q75, 1, 2)).astype(np.int32)
+    materials = {0: dict(kind='isotropic', E=1.0, nu=0.3),
+                 1: dict(kind='orthotropic', frame='local', E1=10.0, E2=2.0, E3=2.0, nu12=0.3, nu13=0.3,
+                         nu23=0.25, G12=1.5, G13=1.5, G23=0.8),
+                 2: dict(kind='isotropic', E=10.0, nu=0.2)}
+    theta = 2 * np.pi * np.arange(n) / n
+    axis1 = np.zeros((n, n, n, 3))
+    axis1[..., 0] = np.cos(theta)[None, None, :]
+    axis1[..., 1] = np.sin(theta)[None, None, :]
+    orientation = {'axis1': axis1, 'axis2': [0.0, 0.0, 1.0]}
+    return phase, materials, orientation
+
+
+def hybrid_form(args):
+    """Average-stress C^H (6 unit-strain solves) of a general-C21 composite through HybridFIOperator."""
+    import pgj
+    from pgj import _lib
+    from pgj.hybrid import HybridFIOperator, enable_fi_reference
+    from pgj.materials import build_hybrid_material
+
+    n = args.n
+    phase, materials, orientation = composite(n, args.seed, args.radius)
+    pool, index, meta = build_hybrid_material(phase, materials, orientation=orientation)
+    grid = (n, n, n)
+    op = enable_fi_reference(HybridFIOperator(pool, index, grid, (1.0 / n,) * 3))
+    for _ in range(max(args.warmup, 1)):
+        C, info = pgj.compute_effective_stiffness_batched(op, tol=1e-8, max_iter=4000)
+    torch.cuda.synchronize()
+    steps = max(1, args.steps // 2)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        C, info = pgj.compute_effective_stiffness_batched(op, tol=1e-8, max_iter=4000)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    elapsed = time.perf_counter() - t_start
+    iters = [info[0][J]['iters'] for J in range(6)]
+    res = max(info[0][J]['rel_res'] for J in range(6))
+    N = n ** 3
+
+    def kernel_us(name, L, reps=20):
+        u = torch.randn(L, 3 * N, dtype=torch.float64, device='cuda')
+        out = torch.empty_like(u)
+        launch = lambda: _lib.call(name, op.grid, L, *op._pool(), _lib.ptr(u), None, _lib.ptr(out), _lib.stream())
+        launch()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            launch()
+            b.record()
+        torch.cuda.synchronize()
+        return 1e3 * float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    t1, t6 = kernel_us('pgj_hybrid_apply', 1), kernel_us('pgj_hybrid_apply', 6)
+    g1 = kernel_us('pgj_hybrid_apply_gather', 1)
+    # algorithmic bytes of one load's apply: u read + K u written (48 N), the row index (4 N) and every pool row
+    pool_bytes = 168 * meta['pool_rows']
+    bytes1 = 52 * N + pool_bytes
+    achieved = bytes1 / t1 / 1e3
+    cpu = None if args.no_cpu else hybrid_cpu_baseline(phase, materials, orientation, sum(iters), args.cpu_seconds)
+    out = {
+        'metric': 'general-C21 average-stress C^H evaluations/s at 64^3 (6 unit-strain solves, rel_res <= 1e-8)',
+        'value': steps / elapsed, 'unit': 'C^H evaluations/s', 'n_gpus': 1, 'steps': steps, 'warmup': 1,
+        'ms_per_step': 1e3 * elapsed / steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (three phases by quartiles of seeded filtered noise)',
+        'config': {'workload': f'{n}^3 three-phase composite: isotropic matrix (E 1, nu 0.3) 50%, local-frame '
+                               'orthotropic fibres (frame turning with x, one pool row per voxel) 25%, isotropic '
+                               'inclusions (E 10, nu 0.2) 25%; zeman reference-medium FI Green preconditioner; '
+                               'solve_pcg_batched eager, tol 1e-8', 'n': n, 'pool_rows': meta['pool_rows']},
+        'latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3)},
+        'iterations': iters, 'max_rel_res': res, 'CH_diag': np.diag(C[0]).tolist(),
+        'apply_us': {'tile_L1': t1, 'tile_L6': t6, 'gather_L1': g1},
+        'apply_fp64_TFLOPs': {'L1': C21_ELEMENT_FLOPS * N / t1 / 1e6, 'L6': C21_ELEMENT_FLOPS * N * 6 / t6 / 1e6},
+        'roofline': {'bound': 'hbm', 'kernel': 'c21_tile_kernel (one load)', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'flop_per_byte': C21_ELEMENT_FLOPS * N / bytes1},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def hybrid_cpu_baseline(phase, materials, orientation, total_iterations, seconds):
+    """The oracle's general-C21 apply on the first z-planes of the same composite, extrapolated to the
+    applies of the GPU run's solves (a lower bound on the CPU time: preconditioner and vector work omitted)."""
+    from threadpoolctl import threadpool_info
+    from oracle import hybrid
+    from pgj.materials import build_hybrid_material
+    n = phase.shape[0]
+    planes = 8
+    sub = np.ascontiguousarray(phase[:planes])
+    orient = {'axis1': np.asarray(orientation['axis1'])[:planes], 'axis2': orientation['axis2']}
+    pool, index, _ = build_hybrid_material(sub, materials, orientation=orient, device='cpu')
+    pool, index = pool.numpy(), index[0].numpy()
+    grid = sub.shape
+    u = np.random.default_rng(0).standard_normal((1, 3 * sub.size))
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        hybrid.apply(pool, index, u, grid, (1.0 / n,) * 3)
+        reps += 1
+    per_node = (time.perf_counter() - t0) / (reps * sub.size)
+    per_ch = per_node * n ** 3 * (total_iterations + 12)
+    threads = max([i.get('num_threads', 1) for i in threadpool_info()] or [1])
+    return {'value': 1.0 / per_ch, 'unit': 'C^H evaluations/s', 'cores': int(threads), 'kind': 'port',
+            'sample': f'{reps} oracle general-C21 applies on the first {planes} z-planes of the composite '
+                      f'({per_node * 1e6:.2f} us per node), extrapolated to {n}^3 and the GPU run\'s '
+                      f'{total_iterations} PCG iterations + 12 rhs/stress-equivalent applies; preconditioner '
+                      f'and vector work omitted, so the CPU figure is an upper bound on its rate'}
+
+
 def design_steps_compare(args):
     """The default design-step workload (64^3, Emin 1e-6, warm starts, OC) with plain FP64 and mixed solves."""
     import pgj
@@ -406,6 +525,9 @@ def main():
     if args.form == 'mixed':
         torch.cuda.set_device(0)
         return mixed_form(args)
+    if args.form == 'hybrid':
+        torch.cuda.set_device(0)
+        return hybrid_form(args)
     if args.profile_stages:
         torch.cuda.set_device(0)
         return profile_stages(args)

@@ -358,11 +358,17 @@ int pgj_pcg_post_step(int B, int64_t n, const double* r, double* z, const double
  * the dense ablation). Only the grid's extents and voxel sizes are used. */
 
 /* out_l = K(C) u_l for L AoS (L, 3N) loads; active (uint8[L], nullable)
- * zeroes inactive loads. A per-node gather over the eight elements in a
- * fixed order (no atomics). Replaces `apply`
+ * zeroes inactive loads. Each block owns a 16 x 16 node column over a z-chunk
+ * and evaluates every element once (plus a one-element halo); node sums are
+ * formed in a fixed order in LDS (no atomics). Replaces `apply`
  * (solver/src/materials/hybrid_kernels.cu:25-48, host :117-127). */
 int pgj_hybrid_apply(const pgj_grid* g, int L, const double* pool, const int32_t* index, const double* u,
                      const uint8_t* active, double* out, void* stream);
+/* The same K by an independent thread-per-node gather that evaluates each of
+ * the node's eight elements (an eight-fold compute redundancy); a second code
+ * path for certification and for checking pgj_hybrid_apply. */
+int pgj_hybrid_apply_gather(const pgj_grid* g, int L, const double* pool, const int32_t* index, const double* u,
+                            const uint8_t* active, double* out, void* stream);
 /* f = -V sum_e Bbar_e^T C_e eps(macro), (1, 3N); macro is 6 host doubles.
  * Replaces `rhs` (solver/src/materials/hybrid_kernels.cu:49-68, host :128-137). */
 int pgj_hybrid_rhs(const pgj_grid* g, const double* pool, const int32_t* index, const double macro[6], double* out,

@@ -58,10 +58,16 @@ def apply(pool, index, u, grid, voxel=None, active=None):
     voxel = _h.voxel_of(grid) if voxel is None else voxel
     u = np.atleast_2d(np.asarray(u, float))
     L = u.shape[0]
-    Ke = element_matrices(element_C(pool, index, grid), voxel)
-    local = _h._gather(u.reshape(L, *grid, 3), grid)
-    forces = np.einsum('...ij,l...j->l...i', Ke, local)
-    out = _h._scatter(forces).reshape(L, -1)
+    Ce = element_C(pool, index, grid).reshape(-1, 6, 6)
+    B = _h.gauss_b_matrices(voxel)
+    w = float(np.prod(voxel)) / 8
+    local = _h._gather(u.reshape(L, *grid, 3), grid).reshape(L, -1, 24)
+    forces = np.zeros_like(local)
+    for q in range(8):                       # Ke u = sum_q B_q^T C (B_q u) V/8, without forming Ke
+        eps = local @ B[q].T
+        sig = np.einsum('nst,lnt->lns', Ce, eps)
+        forces += (sig @ B[q]) * w
+    out = _h._scatter(forces.reshape(L, *grid, 24)).reshape(L, -1)
     if active is not None:
         out[~np.asarray(active, bool)] = 0.0
     return out

@@ -533,9 +533,21 @@ int allow_lds(K kernel, size_t bytes) {
   PGJ_REQUIRE((A).f.runtime_ok, "grid (%d,%d,%d) has only the compile-time resident FFT path", (A).nz, (A).ny, \
               (A).nx)
 
+// With fewer plane blocks than CUs (few loads), the xy kernels take one
+// component per block, which triples the grid; the spectrum layout is the same.
+FFTArgs xy_args(const FFTArgs& A, int L) {
+  FFTArgs B = A;
+  if (A.f.comps == 3 && int64_t(A.nz) * L < 256) {
+    B.f.comps = 1;
+    B.f.lds_xy = (A.f.lds_xy - 64 * 8) / 3 + 64 * 8;
+  }
+  return B;
+}
+
 template <int MODE>
-int launch_fxy(FFTArgs& A, int L, hipStream_t s) {
-  PGJ_RUNTIME_FFT(A);
+int launch_fxy(FFTArgs& A0, int L, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A0);
+  FFTArgs A = xy_args(A0, L);
   const FFTGeom& f = A.f;
   dim3 grid(A.nz, f.comps == 3 ? 1 : 3, L);
   if (int e = allow_lds(fxy_kernel<MODE>, f.lds_xy)) return e;
@@ -555,8 +567,9 @@ int launch_fz(FFTArgs& A, int L, hipStream_t s) {
 }
 
 template <int MODE>
-int launch_fxyi(FFTArgs& A, int L, hipStream_t s) {
-  PGJ_RUNTIME_FFT(A);
+int launch_fxyi(FFTArgs& A0, int L, hipStream_t s) {
+  PGJ_RUNTIME_FFT(A0);
+  FFTArgs A = xy_args(A0, L);
   const FFTGeom& f = A.f;
   dim3 grid(A.nz, f.comps == 3 ? 1 : 3, L);
   if (int e = allow_lds(fxyi_kernel<MODE>, f.lds_xy)) return e;

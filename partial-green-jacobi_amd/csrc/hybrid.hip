@@ -5,6 +5,9 @@
 // apply gathers per node over its eight elements (fixed order, no atomics, so
 // results are independent of the launch); RHS and mean stress use the exact
 // centroid gradient of the trilinear field.
+#include <algorithm>
+#include <climits>
+
 #include "hex8_element.h"
 #include "hex8_kernels.h"
 #include "layout.h"
@@ -58,6 +61,149 @@ __global__ __launch_bounds__(256) void c21_apply_kernel(int nz, int ny, int nx, 
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) out[base + 3 * i + d] = acc[d];
+}
+
+// Tiled apply: a block owns a 16 x 16 column of nodes over a z-chunk and marches
+// the element layers below each owned node plane, computing every element
+// once (plus a one-element halo in -x, -y and -z). Threads 0..255 take the
+// elements under the owned nodes, threads 256..288 the halo elements. Per
+// layer: the next node plane is staged in LDS (two planes, by parity), each element writes the forces
+// on its three in-plane neighbour nodes (bottom and top) to LDS, and the
+// owner of a node adds them in a fixed order; the top-plane sums carry to the
+// next layer in registers. Sums are independent of the launch shape.
+constexpr int kTile = 16;
+constexpr int kTileThreads = 320;
+constexpr int kSide = kTile + 2;  // staged node plane: x0-1 .. x0+16
+constexpr int kElem = kTile + 1;  // element columns: x0-1 .. x0+15
+
+__device__ __forceinline__ int modi(int v, int n) {
+  const int r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+__global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, int nx, int zc, ElementCoef coef,
+                                                                const double* pool, const int32_t* index,
+                                                                const double* u, const uint8_t* active,
+                                                                double* out) {
+  __shared__ double np[2][kSide * kSide * 3];  // node planes z and z+1 (by parity)
+  __shared__ double fx[2][kElem * kElem * 9];
+  const int l = blockIdx.z;
+  const int tiles_x = (nx + kTile - 1) / kTile;
+  const int x0 = int(blockIdx.x % tiles_x) * kTile, y0 = int(blockIdx.x / tiles_x) * kTile;
+  const int z0 = int(blockIdx.y) * zc, zend = min(z0 + zc, nz);
+  const int64_t N = int64_t(nx) * ny * nz;
+  const double* ul = u + int64_t(l) * 3 * N;
+  double* ol = out + int64_t(l) * 3 * N;
+  const int t = threadIdx.x;
+  int i = 0, j = 0;
+  bool has = true;
+  if (t < kTile * kTile) {
+    i = t % kTile;
+    j = t / kTile;
+  } else if (t < kTile * kTile + kElem) {
+    i = -1;
+    j = t - kTile * kTile - 1;
+  } else if (t < kTile * kTile + kElem + kTile) {
+    i = t - kTile * kTile - kElem;
+    j = -1;
+  } else {
+    has = false;
+  }
+  has = has && x0 + i < nx && y0 + j < ny;
+  const bool owner = t < kTile * kTile && has;
+  const int gx = modi(x0 + i, nx), gy = modi(y0 + j, ny);
+
+  if (active && !active[l]) {
+    if (owner)
+      for (int z = z0; z < zend; ++z) {
+        const int64_t node = (int64_t(z) * ny + gy) * nx + gx;
+        for (int d = 0; d < 3; ++d) ol[3 * node + d] = 0.0;
+      }
+    return;
+  }
+
+  // staged plane entries of this thread: k = t and k = t + 320 (< 324)
+  constexpr int kPlane = kSide * kSide;
+  auto fetch = [&](int plane, double (&v)[2][3]) {
+    const int pz = modi(plane, nz);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int k = t + s * kTileThreads;
+      if (k < kPlane) {
+        const int64_t node = (int64_t(pz) * ny + modi(y0 - 1 + k / kSide, ny)) * nx + modi(x0 - 1 + k % kSide, nx);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[s][d] = ul[3 * node + d];
+      }
+    }
+  };
+  auto stage = [&](int plane, const double (&v)[2][3]) {
+    double* b = np[plane & 1];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int k = t + s * kTileThreads;
+      if (k < kPlane)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) b[3 * k + d] = v[s][d];
+    }
+  };
+  // element nodes of one plane, local order x + 2y
+  auto read_nodes = [&](int plane, double* w) {
+    const double* b = np[plane & 1];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int k = (j + 1 + (n >> 1)) * kSide + (i + 1 + (n & 1));
+#pragma unroll
+      for (int d = 0; d < 3; ++d) w[3 * n + d] = b[3 * k + d];
+    }
+  };
+
+  double pre[2][3] = {};
+  fetch(z0 - 1, pre);
+  stage(z0 - 1, pre);
+  fetch(z0, pre);
+  stage(z0, pre);
+  __syncthreads();
+
+  const int slot = (j + 1) * kElem + (i + 1);
+  double carry[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int z = z0 - 1; z < zend; ++z) {
+    const bool more = z + 1 < zend;
+    if (more) fetch(z + 2, pre);
+    double fe[24];
+    if (has) {
+      double ue[24], C[6][6];
+      read_nodes(z, ue);
+      read_nodes(z + 1, ue + 12);
+      const int64_t e = (int64_t(modi(z, nz)) * ny + gy) * nx + gx;
+      unpack_c21(c21_row(pool, index, e), C);
+      hex8_element_c(ue, C, coef, fe);
+#pragma unroll
+      for (int n = 1; n < 4; ++n)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          fx[0][slot * 9 + 3 * (n - 1) + d] = fe[3 * n + d];
+          fx[1][slot * 9 + 3 * (n - 1) + d] = fe[3 * (n + 4) + d];
+        }
+    }
+    __syncthreads();
+    if (owner) {
+      const int sx = slot - 1, sy = slot - kElem, sxy = slot - kElem - 1;
+      double bottom[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        bottom[d] = carry[d] + fe[d] + fx[0][sx * 9 + d] + fx[0][sy * 9 + 3 + d] + fx[0][sxy * 9 + 6 + d];
+        carry[d] = fe[12 + d] + fx[1][sx * 9 + d] + fx[1][sy * 9 + 3 + d] + fx[1][sxy * 9 + 6 + d];
+      }
+      if (z >= z0) {
+        const int64_t node = (int64_t(z) * ny + gy) * nx + gx;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ol[3 * node + d] = bottom[d];
+      }
+    }
+    if (more) stage(z + 2, pre);  // over plane z, which every thread has read before the barrier
+    __syncthreads();
+  }
 }
 
 // f = -sum_e V Bbar_e^T C_e E (the element force of the affine field of E), AoS (1, 3N).
@@ -140,6 +286,26 @@ __global__ void c21_stress_finish(const double* part, int nblk, int64_t N, doubl
 using namespace pgj;
 
 namespace {
+// Blocks of the tiled apply resident at once on the current device (CUs x blocks per CU), cached per device.
+int tile_slots(int* out) {
+  static int cache[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess && (dev < 0 || dev >= 64)) e = hipErrorInvalidDevice;
+  if (e == hipSuccess && cache[dev] == 0) {
+    int cus = 0, per_cu = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c21_tile_kernel, kTileThreads, 0);
+    if (e == hipSuccess) cache[dev] = std::max(1, cus * std::max(1, per_cu));
+  }
+  if (e != hipSuccess) {
+    set_error("device query for the tiled apply failed: %s", hipGetErrorString(e));
+    return PGJ_ERR_HIP;
+  }
+  *out = cache[dev];
+  return PGJ_OK;
+}
+
 int c21_blocks(const pgj_grid& g) { return int((int64_t(g.nx) * g.ny * g.nz + 255) / 256); }
 Strain6 macro_of(const double* m) {
   Strain6 E{};
@@ -150,6 +316,30 @@ Strain6 macro_of(const double* m) {
 
 extern "C" int pgj_hybrid_apply(const pgj_grid* g, int L, const double* pool, const int32_t* index, const double* u,
                                 const uint8_t* active, double* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && pool && u && out, "invalid hybrid apply arguments");
+  const int tiles = ((g->nx + kTile - 1) / kTile) * ((g->ny + kTile - 1) / kTile);
+  // z-chunk with the fewest layer steps on the critical path: rounds of resident blocks x (chunk + halo layer)
+  int slots = 0;
+  if (int e = tile_slots(&slots)) return e;
+  int zc = 2;
+  int64_t best = INT64_MAX;
+  for (int c = 2; c <= 32; c *= 2) {
+    const int64_t blocks = int64_t(tiles) * L * ((g->nz + c - 1) / c);
+    const int64_t cost = ((blocks + slots - 1) / slots) * (std::min(c, g->nz) + 1);
+    if (cost < best) {
+      best = cost;
+      zc = c;
+    }
+  }
+  hipLaunchKernelGGL(c21_tile_kernel, dim3(tiles, (g->nz + zc - 1) / zc, L), dim3(kTileThreads), 0,
+                     as_stream(stream), g->nz, g->ny, g->nx, zc, coef_of(*g), pool, index, u, active, out);
+  PGJ_LAUNCHED("c21_tile_kernel");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_hybrid_apply_gather(const pgj_grid* g, int L, const double* pool, const int32_t* index,
+                                       const double* u, const uint8_t* active, double* out, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(L >= 1 && L <= 65535 && pool && u && out, "invalid hybrid apply arguments");
   hipLaunchKernelGGL(c21_apply_kernel, dim3(c21_blocks(*g), L), dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,

@@ -72,6 +72,7 @@ SIGNATURES = {
     'pgj_alpha_ws_bytes': (c_size_t, [GP]),
     'pgj_alpha_select': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_hybrid_apply': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hybrid_apply_gather': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_hybrid_rhs': (c_int, [GP, c_void_p, c_void_p, c_dp, c_void_p, c_void_p]),
     'pgj_hybrid_stress_ws_bytes': (c_size_t, [GP]),
     'pgj_hybrid_avg_stress': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_dp, c_void_p, c_void_p, c_void_p]),

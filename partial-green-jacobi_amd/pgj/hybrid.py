@@ -115,6 +115,18 @@ class HybridFIOperator:
                   _lib.ptr(out), _lib.stream())
         return out
 
+    def apply_gather(self, u, active=None):
+        """The same K u_l by the independent thread-per-node gather kernel (certification path)."""
+        if u.dim() != 2 or u.shape[1] != self.mesh.n_dofs or u.dtype != self.dtype or u.device != self.device \
+                or not u.is_contiguous():
+            raise ValueError('Expected contiguous FP64 displacements (L,3N) on the material device')
+        if active is not None:
+            active = active.to(device=self.device, dtype=torch.uint8).contiguous()
+        out = torch.empty_like(u)
+        _lib.call('pgj_hybrid_apply_gather', self.grid, u.shape[0], *self._pool(), _lib.ptr(u), _lib.ptr(active),
+                  _lib.ptr(out), _lib.stream())
+        return out
+
     def compute_rhs(self, macro):
         """f = -V sum_e Bbar^T C_e eps(macro), (1, 3N) (``hybrid_adapter.py:92-94``)."""
         m = self._macro_strain_tensor(macro)

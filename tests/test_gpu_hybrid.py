@@ -45,7 +45,8 @@ def multiphase(grid, seed, voxel=None):
     return rng, pool, index, vox
 
 
-CASES = [((6, 6, 6), None), ((3, 4, 5), (0.7, 1.1, 1.4)), ((8, 6, 10), None), ((5, 7, 4), None)]
+CASES = [((6, 6, 6), None), ((3, 4, 5), (0.7, 1.1, 1.4)), ((8, 6, 10), None), ((5, 7, 4), None),
+         ((2, 2, 2), None), ((20, 5, 17), None), ((9, 18, 33), None), ((3, 40, 2), None)]
 
 
 @pytest.mark.parametrize('dense', [False, True], ids=['indirect', 'dense'])
@@ -66,6 +67,7 @@ def test_apply_rhs_stress_match_oracle(grid, voxel, dense):
     ref = hybrid.apply(pool_h, index_h, u, grid, vox)
     batch = op.apply_batched(ut).cpu().numpy()
     assert rel(batch, ref) < 1e-13
+    assert rel(op.apply_gather(ut).cpu().numpy(), ref) < 1e-13
     one = op.apply(ut[1:2].contiguous()).cpu().numpy()
     assert np.array_equal(one[0], batch[1])
     masked = op.apply_batched(ut, torch.tensor([1, 0, 1], device='cuda')).cpu().numpy()
@@ -76,6 +78,21 @@ def test_apply_rhs_stress_match_oracle(grid, voxel, dense):
         assert rel(s, hybrid.avg_stress(pool_h, index_h, u[:1], m, grid, vox)) < 1e-13
     p = op.project_zero_mean(ut[:1]).cpu().numpy()
     assert rel(p, hex8.project(u[:1])) < 1e-13
+
+
+@pytest.mark.parametrize('grid', [(64, 64, 64), (48, 32, 40), (17, 33, 16)], ids=str)
+def test_tiled_apply_equals_gather_at_size(grid):
+    """Tiled and gather kernels at full size, with z-chunking and ragged tiles, several loads and a mask."""
+    rng, pool, index, vox = multiphase(grid, 17)
+    op = HybridFIOperator(pool, index, grid, vox)
+    u = torch.tensor(rng.normal(size=(6, 3 * int(np.prod(grid)))), device='cuda')
+    for L in (6, 1):
+        a = op.apply_batched(u[:L].contiguous()).cpu().numpy()
+        b = op.apply_gather(u[:L].contiguous()).cpu().numpy()
+        assert rel(a, b) < 1e-13
+    mask = torch.tensor([1, 1, 0, 1, 0, 1], device='cuda')
+    m = op.apply_batched(u, mask).cpu().numpy()
+    assert (m[[2, 4]] == 0).all() and np.array_equal(m[[0, 1, 3, 5]], op.apply_batched(u).cpu().numpy()[[0, 1, 3, 5]])
 
 
 @pytest.mark.parametrize('n', [4, 32, 64])
