@@ -56,6 +56,12 @@ int pgj_hex8_apply(const pgj_grid* g, int L, const double* a, const double* u, c
  * exp/asg_gpu_resident_20261103/resident.py:100-102). */
 int pgj_hex8_apply_reference(const pgj_grid* g, int L, const double* a, const double* u, double* out,
                              void* stream);
+/* FP32 drop-in apply: the row-march on FP32 a, u and out (FP32 element
+ * arithmetic), for the inner operator of iterative refinement
+ * (solver/src/runtime/fi/source/femlib/batched/ir.py:33-35, op_inner in FP32).
+ * Needs a grid the row-march supports. */
+int pgj_hex8_apply32(const pgj_grid* g, int L, const float* a, const float* u, const uint8_t* active, float* out,
+                     void* stream);
 
 /* f = -sum_e a_e Ke chi(macro), shape (1, 3N); macro is 6 host doubles in
  * Voigt order with engineering shear. Replaces `rhs`
@@ -348,6 +354,19 @@ int pgj_pcg_pre_step(int B, int64_t n, double* u, double* r, const double* p, co
  * pcg.py:495-508); minv_stride is 0 (one (n) diagonal for every batch) or n. */
 int pgj_pcg_post_step(int B, int64_t n, const double* r, double* z, const double* minv, int64_t minv_stride,
                       double* p, double* rz, double* ws, void* stream);
+/* The same four steps on FP32 vectors (u, r, p, Ap, z, best_u, minv); sums,
+ * rz, rhs_norm, res and best_res stay FP64. The inner PCG of
+ * solve_ir_batched (solver/src/runtime/fi/source/femlib/batched/ir.py:84-92)
+ * runs on these. */
+int pgj_pcg32_start(int B, int64_t n, const float* rhs, const float* Ax, float* r, double* rhs_norm, double* ws,
+                    void* stream);
+int pgj_pcg32_init(int B, int64_t n, const float* r, const float* z, float* p, double* rz, double* ws, void* stream);
+int pgj_pcg32_pre_step(int B, int64_t n, float* u, float* r, const float* p, const float* Ap, const double* rz,
+                       const double* rhs_norm, float* best_u, double* best_res, double* res, uint8_t* improved,
+                       int64_t* state, double tol, double near_tol, int64_t iteration, int latched, double* ws,
+                       void* stream);
+int pgj_pcg32_post_step(int B, int64_t n, const float* r, float* z, const float* minv, int64_t minv_stride, float* p,
+                        double* rz, double* ws, void* stream);
 
 /* ---- general-C21 material (hybrid storage) ----
  * Element e has the stiffness C_e given by its 21 upper-triangle Voigt

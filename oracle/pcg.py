@@ -6,9 +6,13 @@
 grouping of ``exp/asg_sctop_reproduction_20261105/reproduction.py:144-151``.
 ``solve_pcg_batched`` follows ``solver/src/runtime/fi/source/femlib/batched/pcg.py:20-252``
 (eager and compiled/latched update modes, best-iterate return) and
-``backend_solve`` follows ``exp/asg_topopt_20261103/backend.py:123-144``.
+``backend_solve`` follows ``exp/asg_topopt_20261103/backend.py:123-144`` and
+``solve_ir_batched`` follows ``solver/src/runtime/fi/source/femlib/batched/ir.py:23-247``.
 Pinned in ``tests/test_oracle_pcg.py`` against dense direct solves, exact
-zero-RHS behaviour, the iteration cap and the restart/RuntimeError paths.
+zero-RHS behaviour, the iteration cap and the restart/RuntimeError paths; the
+refinement against the dense solution and the closed-form contraction of an
+inner operator c K (outer residual ratio 1 - 1/c: the tightening and stall
+schedule follows from it).
 """
 import math
 
@@ -201,6 +205,62 @@ def solve_pcg_batched(apply, apply_precond, rhs, *, tol=1e-6, max_iter=2000, x0=
         p = z + beta[:, None] * p
         rz = rz_new
     return best_u, n_iter_final, best_res
+
+
+def solve_ir_batched(apply_outer, apply_inner, precond_inner, project, rhs, *, outer_tol=1e-8, inner_tol=0.1,
+                     max_outer=30, max_inner=80, inner_tols=None, adaptive_max_inner=160, stagnation_factor=0.3,
+                     stall_factor=0.9, stall_patience=3):
+    """Iterative refinement (``solver/src/runtime/fi/source/femlib/batched/ir.py:23-107``; with
+    ``inner_tols`` the adaptive variant ``ir.py:110-247``). Inner solves are FP64 eager
+    ``solve_pcg_batched`` on ``apply_inner``. Returns (best_u, info) with ``outer_iters``,
+    ``inner_iters``, ``rel_res``, ``stop_reason`` and (adaptive) the per-step ``history``."""
+    rhs = np.atleast_2d(np.asarray(rhs, float))
+    B = rhs.shape[0]
+    adaptive = inner_tols is not None
+    tols = tuple(inner_tols) if adaptive else (inner_tol,)
+    u = np.zeros_like(rhs)
+    rhs_norm = np.maximum(np.sqrt((rhs * rhs).sum(1)), 1e-30)
+    best_res = np.full(B, np.inf)
+    best_u = u.copy()
+    total, last_k, prev, ti, cur_max, stall, stop = 0, 0, None, 0, int(max_inner), 0, 'max_outer'
+    history = []
+    for k in range(max_outer):
+        r = rhs - apply_outer(u)
+        res = np.sqrt((r * r).sum(1)) / rhs_norm
+        last_k = k
+        imp = res < best_res
+        best_u = np.where(imp[:, None], u, best_u)
+        best_res = np.where(imp, res, best_res)
+        if res.max() < outer_tol:
+            stop = 'converged'
+            break
+        tightened = False
+        if adaptive:
+            if prev is not None and res.max() > stagnation_factor * prev:
+                if ti < len(tols) - 1:
+                    ti += 1
+                    cur_max = max(cur_max, int(adaptive_max_inner))
+                    tightened = True
+                    stall = 0
+                elif res.max() > stall_factor * prev:
+                    stall += 1
+                else:
+                    stall = 0
+            else:
+                stall = 0
+            if stall >= stall_patience:
+                stop = 'stalled'
+                break
+        du, n_in, inner_res = solve_pcg_batched(apply_inner, precond_inner, r, tol=tols[ti], max_iter=cur_max)
+        total += int(n_in)
+        history.append(dict(outer_iter=k, inner_tol=tols[ti], inner_max_iter=cur_max, inner_iters=int(n_in),
+                            tightened=tightened, stall_count=stall))
+        u = project(u + du)
+        prev = float(res.max())
+    if best_res.max() < outer_tol:
+        stop = 'converged'
+    return best_u, dict(outer_iters=last_k + 1, inner_iters=total, rel_res=best_res, stop_reason=stop,
+                        history=history)
 
 
 def backend_solve(apply, apply_precond, rhs, n, x0=None, rtol=1e-8, maxiter=4000):

@@ -844,6 +844,16 @@ extern "C" int pgj_hex8_apply(const pgj_grid* g, int L, const double* a, const d
   return launch_rowmarch(*g, L, kModeApply, A, as_stream(stream));
 }
 
+extern "C" int pgj_hex8_apply32(const pgj_grid* g, int L, const float* a, const float* u, const uint8_t* active,
+                                float* out, void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && a && u && out, "invalid apply arguments");
+  PGJ_REQUIRE(rowmarch_config(*g, L, 4).valid, "the FP32 apply needs the row-march kernel (nx <= 128 and nx %% ceil(nx/64) == 0)");
+  RowArgsT<float> A{};
+  A.a = a; A.u = u; A.out = out; A.mask = active;
+  return launch_rowmarch(*g, L, kModeApply, A, as_stream(stream));
+}
+
 extern "C" int pgj_hex8_apply_reference(const pgj_grid* g, int L, const double* a, const double* u, double* out,
                                         void* stream) {
   if (int e = check_grid(g)) return e;

@@ -30,6 +30,7 @@ SIGNATURES = {
     'pgj_version': (c_int, []),
     'pgj_last_error': (ctypes.c_char_p, []),
     'pgj_hex8_apply': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hex8_apply32': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_hex8_apply_reference': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_hex8_rhs': (c_int, [GP, c_void_p, c_dp, c_void_p, c_void_p]),
     'pgj_hex8_energy': (c_int, [GP, c_int, c_void_p, c_dp, c_void_p, c_void_p]),
@@ -117,6 +118,13 @@ SIGNATURES = {
                                  c_int, c_void_p, c_void_p]),
     'pgj_pcg_post_step': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
+    'pgj_pcg32_start': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_pcg32_init': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_pcg32_pre_step': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int64,
+                                   c_int, c_void_p, c_void_p]),
+    'pgj_pcg32_post_step': (c_int, [c_int, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -26,9 +26,15 @@ def _f64(x, device):
 
 
 class Operator:
-    """B=1 periodic Q1/HEX8 operator for C(x) = a(x) C0(lam, mu) on a CUDA device."""
+    """B=1 periodic Q1/HEX8 operator for C(x) = a(x) C0(lam, mu) on a CUDA device.
 
-    def __init__(self, a, voxel=None, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda'):
+    ``dtype=torch.float32`` gives the inner operator of iterative refinement
+    (``solver/src/runtime/fi/source/femlib/batched/ir.py:33-35``): ``apply``
+    runs the FP32 row-march on an FP32 copy of a; the preconditioner, rhs,
+    stress and projection compute in FP64 and return the operator's dtype.
+    """
+
+    def __init__(self, a, voxel=None, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda', dtype=torch.float64):
         a = _f64(a, device)
         if a.dim() != 3 or min(a.shape) < 2:
             raise ValueError('Periodic grid dimensions >=2 required')
@@ -37,7 +43,9 @@ class Operator:
         if min(voxel) <= 0:
             raise ValueError('positive voxel sizes required')
         self.B = 1
-        self.device, self.dtype = a.device, torch.float64
+        if dtype not in (torch.float64, torch.float32):
+            raise ValueError('operator dtype must be float64 or float32')
+        self.device, self.dtype = a.device, dtype
         if self.device.type != 'cuda':
             raise ValueError('pgj operator requires a CUDA (HIP) device')
         n = nx * ny * nz
@@ -46,6 +54,7 @@ class Operator:
         self.lam, self.mu = float(lam), float(mu)
         self.grid = _lib.make_grid(a.shape, voxel, lam, mu)
         self._material_field = a
+        self._material32 = a.float().contiguous() if dtype == torch.float32 else None
         self._voxel = voxel
         self._diagonal = None
         self._log_diagonal = None
@@ -61,25 +70,32 @@ class Operator:
     def _validate_apply_input(self, u):
         if u.shape != (1, self.mesh.n_dofs) or u.dtype != self.dtype or u.device != self.device \
                 or not u.is_contiguous():
-            raise ValueError('Expected contiguous FP64 displacement (1,3N) on the material device')
+            name = 'FP64' if self.dtype == torch.float64 else 'FP32'
+            raise ValueError(f'Expected contiguous {name} displacement (1,3N) on the material device')
+
+    def _apply_call(self, L, u, active, out):
+        if self.dtype == torch.float32:
+            _lib.call('pgj_hex8_apply32', self.grid, L, _lib.ptr(self._material32), _lib.ptr(u), _lib.ptr(active),
+                      _lib.ptr(out), _lib.stream())
+        else:
+            _lib.call('pgj_hex8_apply', self.grid, L, _lib.ptr(self._material_field), _lib.ptr(u), _lib.ptr(active),
+                      _lib.ptr(out), _lib.stream())
 
     def apply(self, u):
         """K(a) u for one (1, 3N) displacement (``hybrid_adapter.py:87-90``)."""
         self._validate_apply_input(u)
         out = torch.empty_like(u)
-        _lib.call('pgj_hex8_apply', self.grid, 1, _lib.ptr(self._material_field), _lib.ptr(u), None,
-                  _lib.ptr(out), _lib.stream())
+        self._apply_call(1, u, None, out)
         return out
 
     def apply_batched(self, u, active=None):
         """K(a) u_l for (L, 3N) loads; inactive loads give zeros (``scalar_fi.cu:46-53``)."""
         if u.dim() != 2 or u.shape[1] != self.mesh.n_dofs or u.dtype != self.dtype or not u.is_contiguous():
-            raise ValueError('Expected contiguous FP64 displacements (L,3N)')
+            raise ValueError('Expected contiguous displacements (L,3N) of the operator dtype')
         out = torch.empty_like(u)
         if active is not None:
             active = active.to(torch.uint8).contiguous()
-        _lib.call('pgj_hex8_apply', self.grid, u.shape[0], _lib.ptr(self._material_field), _lib.ptr(u),
-                  _lib.ptr(active), _lib.ptr(out), _lib.stream())
+        self._apply_call(u.shape[0], u, active, out)
         return out
 
     def apply_reference(self, u):
@@ -97,30 +113,31 @@ class Operator:
     def compute_rhs(self, macro):
         """f = -sum_e a_e Ke chi(macro), (1, 3N) (``hybrid_adapter.py:92-94``)."""
         m = macro.tolist() if torch.is_tensor(macro) else macro
-        out = torch.empty((1, self.mesh.n_dofs), dtype=self.dtype, device=self.device)
+        out = torch.empty((1, self.mesh.n_dofs), dtype=torch.float64, device=self.device)
         _lib.call('pgj_hex8_rhs', self.grid, _lib.ptr(self._material_field), self._macro(m), _lib.ptr(out),
                   _lib.stream())
-        return out
+        return out.to(self.dtype)
 
     def compute_avg_stress(self, u, macro):
         """Mean stress <C(eps + B u)> as numpy (1, 6) (``hybrid_adapter.py:96-99``)."""
         self._validate_apply_input(u)
         m = macro.tolist() if torch.is_tensor(macro) else macro
-        out = torch.empty(6, dtype=self.dtype, device=self.device)
+        u = u.to(torch.float64)
+        out = torch.empty(6, dtype=torch.float64, device=self.device)
         _lib.call('pgj_hex8_avg_stress', self.grid, _lib.ptr(self._material_field), _lib.ptr(u), self._macro(m),
                   _lib.ptr(self._stress_ws), _lib.ptr(out), _lib.stream())
         return out.cpu().numpy().reshape(1, 6)
 
     def project_zero_mean(self, value):
         """Zero-mean projection per component (``hybrid_adapter.py:78-80``); returns a new tensor."""
-        out = value.reshape(value.shape[0], -1).contiguous().clone()
+        out = value.reshape(value.shape[0], -1).to(torch.float64).contiguous().clone()
         L = out.shape[0]
         ws = self._proj_ws
         need = _lib.lib().pgj_project_ws_bytes(self.grid, L) // 8 + 1
         if ws.numel() < need:
             ws = self._proj_ws = torch.empty(need, dtype=torch.float64, device=self.device)
         _lib.call('pgj_project', self.grid, L, _lib.ptr(out), _lib.ptr(ws), _lib.stream())
-        return out.reshape(value.shape)
+        return out.reshape(value.shape).to(value.dtype)
 
     # -- spectral Green part -------------------------------------------------------------
     def enable_green(self):
@@ -159,20 +176,21 @@ class Operator:
         return shat
 
     def green_precond(self, r, s=None, shat=None):
-        """z = P s G s P r for (L, 3N) residuals (s None: P G P r)."""
-        r = r.reshape(r.shape[0], -1).contiguous()
+        """z = P s G s P r for (L, 3N) residuals (s None: P G P r), in FP64; FP32 residuals come back FP32."""
+        dt = r.dtype
+        r = r.reshape(r.shape[0], -1).to(torch.float64).contiguous()
         z = torch.empty_like(r)
         _lib.call('pgj_precond_apply', self.grid, r.shape[0], _lib.ptr(self._plan), _lib.ptr(r),
                   _lib.ptr(s), _lib.ptr(shat), _lib.ptr(z), _lib.ptr(self._workspace(r.shape[0])), _lib.stream())
-        return z
+        return z.to(dt)
 
     def _unscaled_green(self, r):
         return self.green_precond(r)
 
 
-def build(rho, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda'):
+def build(rho, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda', dtype=torch.float64):
     """Operator + FP64 FI Green with alpha = 0 attached (``adaptive_green_scaling/study.py:61-77``)."""
-    op = Operator(rho, None, lam, mu, device).enable_green()
+    op = Operator(rho, None, lam, mu, device, dtype).enable_green()
     set_alpha(op, 0.0)
     return op
 

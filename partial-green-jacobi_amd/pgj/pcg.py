@@ -19,7 +19,9 @@ launch. The update modes differ as in the reference:
 * ``triton``, ``cuda``: Jacobi only, z = M^-1 r fused into the r.z reduction
   (``pcg.py:297-455``).
 
-Tensors must be FP64 and on the HIP device; there is no host fallback.
+Tensors must be FP64 (or FP32 throughout, for the inner solve of iterative
+refinement: vectors FP32, sums and scalars FP64) and on the HIP device; there
+is no host fallback.
 """
 import math
 import operator as _operator
@@ -68,19 +70,21 @@ def _make_precond(op):
     return lambda r: cache * r
 
 
-def _vector(t, shape, what):
-    if not torch.is_tensor(t) or t.dtype != torch.float64 or t.device.type != 'cuda':
-        raise ValueError(f'{what} must be an FP64 tensor on the HIP device')
+def _vector(t, shape, what, dtype=torch.float64):
+    if not torch.is_tensor(t) or t.dtype != dtype or t.device.type != 'cuda':
+        name = 'FP64' if dtype == torch.float64 else 'FP32'
+        raise ValueError(f'{what} must be an {name} tensor on the HIP device')
     if tuple(t.shape) != tuple(shape):
         t = t.reshape(shape)
     return t.contiguous()
 
 
 class _Workspace:
-    """Device buffers of one solve: partial rows and per-batch scalars."""
+    """Device buffers of one solve: partial rows and per-batch scalars (FP64 for both vector precisions)."""
 
-    def __init__(self, B, n, device):
+    def __init__(self, B, n, device, dtype=torch.float64):
         self.B, self.n = B, n
+        self.pre = 'pgj_pcg_' if dtype == torch.float64 else 'pgj_pcg32_'
         self.ws = torch.empty(_lib.lib().pgj_pcg_ws_bytes(B, n) // 8 + 1, dtype=torch.float64, device=device)
         self.rhs_norm = torch.empty(B, dtype=torch.float64, device=device)
         self.rz = torch.empty(B, dtype=torch.float64, device=device)
@@ -90,39 +94,39 @@ class _Workspace:
         self.state = torch.zeros(2, dtype=torch.int64, device=device)
 
     def start(self, rhs, ax, r):
-        _lib.call('pgj_pcg_start', self.B, self.n, _lib.ptr(rhs), _lib.ptr(ax), _lib.ptr(r), _lib.ptr(self.rhs_norm),
+        _lib.call(self.pre + 'start', self.B, self.n, _lib.ptr(rhs), _lib.ptr(ax), _lib.ptr(r), _lib.ptr(self.rhs_norm),
                   _lib.ptr(self.ws), _lib.stream())
 
     def init(self, r, z, p):
-        _lib.call('pgj_pcg_init', self.B, self.n, _lib.ptr(r), _lib.ptr(z), _lib.ptr(p), _lib.ptr(self.rz),
+        _lib.call(self.pre + 'init', self.B, self.n, _lib.ptr(r), _lib.ptr(z), _lib.ptr(p), _lib.ptr(self.rz),
                   _lib.ptr(self.ws), _lib.stream())
 
     def pre_step(self, u, r, p, ap, best_u, tol, near_tol, iteration, latched):
-        _lib.call('pgj_pcg_pre_step', self.B, self.n, _lib.ptr(u), _lib.ptr(r), _lib.ptr(p), _lib.ptr(ap),
+        _lib.call(self.pre + 'pre_step', self.B, self.n, _lib.ptr(u), _lib.ptr(r), _lib.ptr(p), _lib.ptr(ap),
                   _lib.ptr(self.rz), _lib.ptr(self.rhs_norm), _lib.ptr(best_u), _lib.ptr(self.best_res),
                   _lib.ptr(self.res), _lib.ptr(self.improved), _lib.ptr(self.state), float(tol), float(near_tol),
                   int(iteration), int(latched), _lib.ptr(self.ws), _lib.stream())
 
     def post_step(self, r, z, p, minv=None):
         stride = 0 if minv is None or minv.numel() == self.n else self.n
-        _lib.call('pgj_pcg_post_step', self.B, self.n, _lib.ptr(r), _lib.ptr(z), _lib.ptr(minv), stride, _lib.ptr(p),
+        _lib.call(self.pre + 'post_step', self.B, self.n, _lib.ptr(r), _lib.ptr(z), _lib.ptr(minv), stride, _lib.ptr(p),
                   _lib.ptr(self.rz), _lib.ptr(self.ws), _lib.stream())
 
     def residual_norms(self, rhs, ax, out):
         """out = rhs - ax (ax None: rhs); returns |out_b| per batch (device tensor)."""
         B, n = self.B, self.n
-        _lib.call('pgj_pcg_start', B, n, _lib.ptr(rhs), _lib.ptr(ax), _lib.ptr(out), _lib.ptr(self.rhs_norm),
+        _lib.call(self.pre + 'start', B, n, _lib.ptr(rhs), _lib.ptr(ax), _lib.ptr(out), _lib.ptr(self.rhs_norm),
                   _lib.ptr(self.ws), _lib.stream())
         rr = torch.empty(B, dtype=torch.float64, device=rhs.device)
         copy = torch.empty_like(out)
-        _lib.call('pgj_pcg_init', B, n, _lib.ptr(out), _lib.ptr(out), _lib.ptr(copy), _lib.ptr(rr),
+        _lib.call(self.pre + 'init', B, n, _lib.ptr(out), _lib.ptr(out), _lib.ptr(copy), _lib.ptr(rr),
                   _lib.ptr(self.ws), _lib.stream())
         return rr.sqrt()
 
 
 def _true_relative_residual(op, rhs, u, rhs_norm, work):
     """|rhs - A u| / rhs_norm per batch (``pcg.py:248-250``)."""
-    ax = _vector(op.apply(u), rhs.shape, 'op.apply(u)')
+    ax = _vector(op.apply(u), rhs.shape, 'op.apply(u)', rhs.dtype)
     return work.residual_norms(rhs, ax, torch.empty_like(rhs)) / rhs_norm
 
 
@@ -147,26 +151,27 @@ def solve_pcg_batched(op, rhs, *, tol=1e-6, max_iter=2000, verbose=False, log_ev
         if not (update_mode == 'compiled' and device.type == 'cuda' and convergence_check_stride > 1):
             raise ValueError("adaptive convergence checks require update_mode='compiled', "
                              'CUDA tensors, and convergence_check_stride > 1')
-    rhs = _vector(rhs, (B, n), 'rhs')
-    work = _Workspace(B, n, device)
+    dt = rhs.dtype if torch.is_tensor(rhs) and rhs.dtype == torch.float32 else torch.float64
+    rhs = _vector(rhs, (B, n), 'rhs', dt)
+    work = _Workspace(B, n, device, dt)
 
     if x0 is None:
         u = torch.zeros_like(rhs)
         r = torch.empty_like(rhs)
         work.start(rhs, None, r)
     else:
-        u = _vector(x0, (B, n), 'x0').clone()
+        u = _vector(x0, (B, n), 'x0', dt).clone()
         r = torch.empty_like(rhs)
-        work.start(rhs, _vector(op.apply(u), (B, n), 'op.apply(x0)'), r)
+        work.start(rhs, _vector(op.apply(u), (B, n), 'op.apply(x0)', dt), r)
     rhs_norm = work.rhs_norm.clone()
 
-    z = _vector(apply_m_inv(r), (B, n), 'apply_precond(r)')
+    z = _vector(apply_m_inv(r), (B, n), 'apply_precond(r)', dt)
     p = torch.empty_like(rhs)
     work.init(r, z, p)
     minv = None
     if jacobi:
         minv = _jacobi_m_inv(op)
-        minv = _vector(minv, minv.shape, 'Jacobi diagonal inverse')
+        minv = _vector(minv, minv.shape, 'Jacobi diagonal inverse', dt)
         if minv.numel() not in (n, B * n):
             minv = torch.broadcast_to(minv, (B, n)).contiguous()
         z = torch.empty_like(rhs)
@@ -180,7 +185,7 @@ def solve_pcg_batched(op, rhs, *, tol=1e-6, max_iter=2000, verbose=False, log_ev
 
     n_iter_final = max_iter
     for it in range(max_iter):
-        ap = _vector(op.apply(p), (B, n), 'op.apply(p)')
+        ap = _vector(op.apply(p), (B, n), 'op.apply(p)', dt)
         if recycle is not None and it % recycle_stride == 0:
             recycle(it, p.clone(), ap)
         work.pre_step(u, r, p, ap, best_u, tol, near_tol, it + 1, latched)
@@ -212,7 +217,7 @@ def solve_pcg_batched(op, rhs, *, tol=1e-6, max_iter=2000, verbose=False, log_ev
         if jacobi:
             work.post_step(r, z, p, minv)
         else:
-            z = _vector(apply_m_inv(r), (B, n), 'apply_precond(r)')
+            z = _vector(apply_m_inv(r), (B, n), 'apply_precond(r)', dt)
             work.post_step(r, z, p)
 
     if verbose:

@@ -110,3 +110,43 @@ def test_generic_pcg_and_backend_solve():
         pcg.solve_pcg_batched(apply, pre, f, update_mode='bogus')
     with pytest.raises(ValueError):
         pcg.solve_pcg_batched(apply, pre, f, convergence_check_stride=0)
+
+
+def _spd_system(seed, n=40, cond=50.0):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    K = q @ np.diag(np.geomspace(1.0, cond, n)) @ q.T
+    return rng, K
+
+
+def test_ir_reaches_the_dense_solution():
+    rng, K = _spd_system(1)
+    f = rng.standard_normal((2, K.shape[0]))
+    u, info = pcg.solve_ir_batched(lambda x: x @ K, lambda x: x @ K, lambda r: r, lambda v: v, f,
+                                   outer_tol=1e-10, inner_tol=0.1, max_inner=200)
+    np.testing.assert_allclose(u, np.linalg.solve(K, f.T).T, rtol=1e-8, atol=1e-9)
+    assert info['stop_reason'] == 'converged' and info['rel_res'].max() < 1e-10
+    # an exact inner solve converges in one correction
+    _, one = pcg.solve_ir_batched(lambda x: x @ K, lambda x: x @ K, lambda r: r, lambda v: v, f,
+                                  outer_tol=1e-8, inner_tol=1e-13, max_inner=400)
+    assert one['outer_iters'] == 2 and one['inner_iters'] > 0
+
+
+@pytest.mark.parametrize('c,stop', [(2.0, 'converged'), (20.0, 'stalled')])
+def test_adaptive_ir_schedule_of_a_scaled_inner_operator(c, stop):
+    """Inner operator c K solved tightly: each correction is K^-1 r / c, so the outer residual
+    contracts by exactly 1 - 1/c. c = 2 (ratio 0.5 > 0.3) tightens at every step until the last
+    tolerance and still converges; c = 20 (ratio 0.95 > 0.9) stalls three steps after the last
+    tightening."""
+    rng, K = _spd_system(2)
+    f = rng.standard_normal((1, K.shape[0]))
+    u, info = pcg.solve_ir_batched(lambda x: x @ K, lambda x: c * (x @ K), lambda r: r, lambda v: v, f,
+                                   outer_tol=1e-8, inner_tols=[1e-12, 1e-12, 1e-12], max_inner=400,
+                                   adaptive_max_inner=400, max_outer=60)
+    h = info['history']
+    assert [e['tightened'] for e in h[:3]] == [False, True, True]
+    assert info['stop_reason'] == stop
+    if stop == 'stalled':
+        assert [e['stall_count'] for e in h[2:]] == [0, 1, 2] and info['outer_iters'] == len(h) + 1
+    else:
+        assert info['outer_iters'] == int(np.ceil(np.log(1e-8) / np.log(0.5))) + 1

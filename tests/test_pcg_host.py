@@ -101,3 +101,23 @@ def test_oracle_adaptive_requires_latched():
     with pytest.raises(ValueError):
         opcg.solve_pcg_batched(lambda u: u, lambda r: r, np.ones((1, 3)), update_mode='eager',
                                adaptive_factor=10.0)
+
+
+class _Shell:
+    def __init__(self, B, dtype=torch.float64):
+        self.B, self.dtype = B, dtype
+
+
+def test_ir_argument_errors_before_device_work():
+    from pgj import ir
+    rhs = torch.zeros(1, 6, dtype=torch.float64)
+    with pytest.raises(AssertionError, match='batch size mismatch'):
+        ir.solve_ir_batched(_Shell(1), _Shell(2, torch.float32), rhs)
+    with pytest.raises(ValueError, match='at least one tolerance'):
+        ir.solve_ir_adaptive_batched(_Shell(1), _Shell(1), rhs, inner_tols=())
+    with pytest.raises(ValueError, match='must be positive'):
+        ir.solve_ir_adaptive_batched(_Shell(1), _Shell(1), rhs, inner_tols=(0.1, 0.0))
+    with pytest.raises(ValueError, match='max_outer'):
+        ir.solve_ir_adaptive_batched(_Shell(1), _Shell(1), rhs, max_outer=0)
+    with pytest.raises(ValueError, match='adaptive_max_inner'):
+        ir.solve_ir_adaptive_batched(_Shell(1), _Shell(1), rhs, adaptive_max_inner=0)
