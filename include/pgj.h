@@ -130,6 +130,12 @@ size_t pgj_plan_bytes(const pgj_grid* g);
 int pgj_plan_init(const pgj_grid* g, void* plan, void* stream);
 /* Copies the symbol out of a plan as (6, nz, ny, nx/2+1) doubles. */
 int pgj_plan_symbol(const pgj_grid* g, const void* plan, double* out, void* stream);
+/* The factored FI Green symbol of the grid's (lam, mu): out4 (4, nz, ny, nx/2+1)
+ * holds D_00, D_11, D_22 and beta with G = D - beta (D v)(D v)^T (no 1/N), and
+ * vtab (nullable) the per-axis v tables, nx/2+1 then ny then nz values.
+ * Replaces `formula.fi_coeff` and the v tables of `enable_fi_reference`
+ * (solver/src/reference_medium/formula_kernels.cu:104-126, formula_adapter.py:49-54). */
+int pgj_fi_coeff(const pgj_grid* g, double* out4, double* vtab, void* stream);
 
 /* Spectrum of the scale s (3, nz, ny, nx/2+1) complex128, needed by the
  * fused projection of M_alpha; recompute whenever s changes. */

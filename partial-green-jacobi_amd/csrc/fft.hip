@@ -452,45 +452,78 @@ __global__ void plan_twiddles(int nx, int ny, int nz, double2* twx, double2* twy
   if (i < nz) twz[i] = w(i, nz);
 }
 
-// Dense FI Green symbol: G = (D - beta (Dv)(Dv)^T)/N, D_ii = 1/(mu t sum(d+v^2) + (lam+mu) t d_i).
-__global__ void plan_symbol(int nz, int ny, int nx, double hx, double hy, double hz, double lam, double mu,
-                            double* sym) {
+// FI Green factors at half-spectrum index k: D_ii = 1/(mu t sum(d+v^2) + (lam+mu) t d_i),
+// beta = (lam+mu) t / (1 + (lam+mu) t sum v_i^2 D_ii), per-axis v_i; zero at k = 0.
+struct FiFactors {
+  double inv[3], v[3], beta;
+};
+
+__device__ FiFactors fi_factors(int64_t k, int nz, int ny, int nx, double hx, double hy, double hz, double lam,
+                                double mu) {
   const int nkx = nx / 2 + 1;
-  const int64_t nf = int64_t(nz) * ny * nkx;
-  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= nf) return;
   const int kx = int(k % nkx), ky = int((k / nkx) % ny), kz = int(k / (int64_t(nkx) * ny));
   const int idx[3] = {kx, ky, kz};
   const int n[3] = {nx, ny, nz};
   const double h[3] = {hx, hy, hz};
-  double m[3], v[3], d[3];
+  FiFactors F;
+  double m[3], d[3];
   double t = hx * hy * hz, trace = 0.0;
   for (int i = 0; i < 3; ++i) {
     double s, c;
     sincospi(double(idx[i]) / double(n[i]), &s, &c);
     m[i] = c * c + s * s / 3.0;
     t *= m[i];
-    v[i] = 2.0 * s * c / (h[i] * m[i]);
+    F.v[i] = 2.0 * s * c / (h[i] * m[i]);
     d[i] = 4.0 * (s * s) * (s * s) / (3.0 * h[i] * h[i] * m[i] * m[i]);
-    trace += d[i] + v[i] * v[i];
+    trace += d[i] + F.v[i] * F.v[i];
   }
   trace *= t;
   const double b = (lam + mu) * t;
-  double inv[3], den = 1.0;
+  double den = 1.0;
   for (int i = 0; i < 3; ++i) {
-    inv[i] = k == 0 ? 0.0 : 1.0 / (mu * trace + (lam + mu) * t * d[i]);
-    den += b * v[i] * v[i] * inv[i];
+    F.inv[i] = k == 0 ? 0.0 : 1.0 / (mu * trace + (lam + mu) * t * d[i]);
+    den += b * F.v[i] * F.v[i] * F.inv[i];
   }
-  const double beta = k == 0 ? 0.0 : b / den;
+  F.beta = k == 0 ? 0.0 : b / den;
+  return F;
+}
+
+// Dense FI Green symbol: G = (D - beta (Dv)(Dv)^T)/N.
+__global__ void plan_symbol(int nz, int ny, int nx, double hx, double hy, double hz, double lam, double mu,
+                            double* sym) {
+  const int64_t nf = int64_t(nz) * ny * (nx / 2 + 1);
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  const FiFactors F = fi_factors(k, nz, ny, nx, hx, hy, hz, lam, mu);
+  const double beta = F.beta;
   const double invN = 1.0 / (double(nx) * ny * nz);
   double w[3];
-  for (int i = 0; i < 3; ++i) w[i] = inv[i] * v[i];
-  sym[k] = (inv[0] - beta * w[0] * w[0]) * invN;
-  sym[nf + k] = (inv[1] - beta * w[1] * w[1]) * invN;
-  sym[2 * nf + k] = (inv[2] - beta * w[2] * w[2]) * invN;
+  for (int i = 0; i < 3; ++i) w[i] = F.inv[i] * F.v[i];
+  sym[k] = (F.inv[0] - beta * w[0] * w[0]) * invN;
+  sym[nf + k] = (F.inv[1] - beta * w[1] * w[1]) * invN;
+  sym[2 * nf + k] = (F.inv[2] - beta * w[2] * w[2]) * invN;
   sym[3 * nf + k] = -beta * w[0] * w[1] * invN;
   sym[4 * nf + k] = -beta * w[0] * w[2] * invN;
   sym[5 * nf + k] = -beta * w[1] * w[2] * invN;
+}
+
+// The factored form (D_00, D_11, D_22, beta) per k, unnormalised, and the per-axis
+// v tables (x over nx/2+1, then y, then z).
+__global__ void fi_coeff_kernel(int nz, int ny, int nx, double hx, double hy, double hz, double lam, double mu,
+                                double* out4, double* vtab) {
+  const int nkx = nx / 2 + 1;
+  const int64_t nf = int64_t(nz) * ny * nkx;
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  const FiFactors F = fi_factors(k, nz, ny, nx, hx, hy, hz, lam, mu);
+  for (int i = 0; i < 3; ++i) out4[i * nf + k] = F.inv[i];
+  out4[3 * nf + k] = F.beta;
+  if (vtab) {
+    const int kx = int(k % nkx), ky = int((k / nkx) % ny), kz = int(k / (int64_t(nkx) * ny));
+    if (ky == 0 && kz == 0) vtab[kx] = F.v[0];
+    if (kx == 0 && kz == 0) vtab[nkx + ky] = F.v[1];
+    if (kx == 0 && ky == 0) vtab[nkx + ny + kz] = F.v[2];
+  }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -621,6 +654,16 @@ extern "C" int pgj_plan_init(const pgj_grid* g, void* plan, void* stream) {
   hipLaunchKernelGGL(plan_symbol, dim3(unsigned((f.nf + 255) / 256)), dim3(256), 0, s, g->nz, g->ny, g->nx, g->hx,
                      g->hy, g->hz, g->lam, g->mu, reinterpret_cast<double*>(b + pl.sym));
   PGJ_LAUNCHED("plan_symbol");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_fi_coeff(const pgj_grid* g, double* out4, double* vtab, void* stream) {
+  if (int e = check_fft(g)) return e;
+  PGJ_REQUIRE(out4, "null output");
+  const FFTGeom f = fft_geom(*g);
+  hipLaunchKernelGGL(fi_coeff_kernel, dim3(unsigned((f.nf + 255) / 256)), dim3(256), 0, as_stream(stream), g->nz,
+                     g->ny, g->nx, g->hx, g->hy, g->hz, g->lam, g->mu, out4, vtab);
+  PGJ_LAUNCHED("fi_coeff_kernel");
   return PGJ_OK;
 }
 

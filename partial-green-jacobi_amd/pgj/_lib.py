@@ -50,6 +50,7 @@ SIGNATURES = {
     'pgj_plan_bytes': (c_size_t, [GP]),
     'pgj_plan_init': (c_int, [GP, c_void_p, c_void_p]),
     'pgj_plan_symbol': (c_int, [GP, c_void_p, c_void_p, c_void_p]),
+    'pgj_fi_coeff': (c_int, [GP, c_void_p, c_void_p, c_void_p]),
     'pgj_precond_ws_bytes': (c_size_t, [GP, c_int]),
     'pgj_scale_spectrum': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_precond_apply': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

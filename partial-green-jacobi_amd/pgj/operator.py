@@ -152,6 +152,47 @@ class Operator:
                                       normalization='folded once into coefficients', native='libpgj LDS Stockham')
         return self
 
+    def _fi_tables(self):
+        if getattr(self, '_fi_cache', None) is None:
+            nz, ny, nx = self.mesh.Nz, self.mesh.Ny, self.mesh.Nx
+            nkx = nx // 2 + 1
+            out4 = torch.empty((4, nz * ny * nkx), dtype=torch.float64, device=self.device)
+            vtab = torch.empty(nkx + ny + nz, dtype=torch.float64, device=self.device)
+            _lib.call('pgj_fi_coeff', self.grid, _lib.ptr(out4), _lib.ptr(vtab), _lib.stream())
+            self._fi_cache = (out4, [vtab[:nkx], vtab[nkx:nkx + ny], vtab[nkx + ny:]])
+        return self._fi_cache
+
+    @property
+    def _fft_symbol_inv(self):
+        """(4, nf) factored symbol D_00, D_11, D_22, beta (``formula_adapter.py:54``)."""
+        return self._fi_tables()[0]
+
+    @property
+    def _formula_vtables(self):
+        """Per-axis v tables [x (nx/2+1), y (ny), z (nz)] (``formula_adapter.py:49-53``)."""
+        return self._fi_tables()[1]
+
+    @property
+    def _contest_plan(self):
+        """The plan buffer (twiddles and the dense symbol) the Green apply reads."""
+        return self._plan
+
+    @property
+    def _hybrid_coeff(self):
+        """(N, 21) general-C21 rows a_e C0 of this scalar field (``hybrid_adapter.py:57``), built on demand."""
+        c0 = np.zeros((6, 6))
+        c0[:3, :3] = self.lam
+        c0[np.arange(3), np.arange(3)] += 2 * self.mu
+        c0[np.arange(3, 6), np.arange(3, 6)] = self.mu
+        row = torch.tensor(c0[np.triu_indices(6)], dtype=torch.float64, device=self.device)
+        return self._material_field.reshape(-1, 1) * row[None]
+
+    @property
+    def _hybrid_index(self):
+        return torch.empty(0, dtype=torch.int32, device=self.device)
+
+    _hybrid_indirect = False
+
     @property
     def _contest_coefficient(self):
         nz, ny, nx = self.mesh.Nz, self.mesh.Ny, self.mesh.Nx
@@ -222,6 +263,9 @@ def set_alpha(op, alpha):
         op._scale = op._scale_hat = None
         op.apply_precond = lambda r: op.green_precond(r)
     else:
+        if op._log_diagonal is None:
+            q = diagonal(op).log()
+            op._log_diagonal = q - q.mean()   # the attribute the reference attaches (study.py:96-98)
         s = scale(op, alpha)
         op._scale, op._scale_hat = s, op.scale_spectrum(s)
         op.apply_precond = lambda r: op.green_precond(r, op._scale, op._scale_hat)

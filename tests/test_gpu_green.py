@@ -86,3 +86,39 @@ def test_static_kernels_match_runtime(n):
     if n <= 48 or n == 128:
         G6 = green.symbol(a.shape, hex8.voxel_of(a.shape), LAM, MU)
         assert rel(z, green.precond(r.cpu().numpy(), G6, a.shape, green.scale(a, 0.5, C0))) < 1e-12
+
+
+@pytest.mark.parametrize('grid', [(8, 8, 8), (5, 6, 7), (4, 6, 9)], ids=str)
+def test_factored_symbol_and_protocol_attributes(grid):
+    """The (D, beta) form and v tables rebuild the plan's dense symbol; the C21 rows a C0 give the same K."""
+    from pgj.hybrid import HybridFIOperator
+    rng = np.random.default_rng(3)
+    a = rng.uniform(0.2, 1.0, grid)
+    op = pgj.build(torch.tensor(a, device='cuda'))
+    D, beta = op._fft_symbol_inv[:3].cpu().numpy(), op._fft_symbol_inv[3].cpu().numpy()
+    vx, vy, vz = (t.cpu().numpy() for t in op._formula_vtables)
+    nz, ny, nx = grid
+    assert vx.shape == (nx // 2 + 1,) and vy.shape == (ny,) and vz.shape == (nz,)
+    v = np.stack(np.broadcast_arrays(vx[None, None, :], vy[None, :, None], vz[:, None, None])).reshape(3, -1)
+    w = D * v
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    G = np.stack([(D[i] if i == j else 0.0) - beta * w[i] * w[j] for i, j in pairs])
+    sym = op._contest_coefficient.cpu().numpy() * a.size
+    assert np.abs(G - sym).max() <= 1e-13 * np.abs(sym).max()
+    assert op._contest_plan is op._plan and op._hybrid_indirect is False
+    hop = HybridFIOperator(op._hybrid_coeff.contiguous(), op._hybrid_index, grid, op._voxel, dense=True)
+    u = torch.tensor(rng.normal(size=(1, 3 * a.size)), device='cuda')
+    ref = op.apply(u).cpu().numpy()
+    assert np.abs(hop.apply(u).cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_set_alpha_attaches_the_centred_log_diagonal():
+    rng = np.random.default_rng(8)
+    a = rng.uniform(0.05, 1.0, (6, 6, 6))
+    op = pgj.build(torch.tensor(a, device='cuda'))
+    pgj.set_alpha(op, 0.5)
+    d = hex8.diagonal(a, C0)
+    q = np.log(d) - np.log(d).mean()
+    assert np.abs(op._log_diagonal.cpu().numpy() - q).max() < 1e-13
+    s = np.exp(-0.25 * q)
+    assert np.abs(op._scale.cpu().numpy() - s).max() < 1e-13
