@@ -47,7 +47,7 @@ def parse():
     p.add_argument('--probe-iters', type=int, default=10)
     p.add_argument('--cpu-seconds', type=float, default=15.0)
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid'), default='plain',
+    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid', 'configs'), default='plain',
                    help='plain: the FP64 drop-in design step (default); batched: 48 designs of uneven grids in one '
                         'captured graph vs the same designs solved one at a time; serve: Poisson-arrival '
                         'design-step requests through the continuous-batching server; mixed: FP32-inner / '
@@ -403,6 +403,50 @@ def hybrid_cpu_baseline(phase, materials, orientation, total_iterations, seconds
                       f'and vector work omitted, so the CPU figure is an upper bound on its rate'}
 
 
+def configs_form(args):
+    """Cold certified 6-load solves: 16^3 (Emin 1e-4, box radius 2, alpha 0.5, serial) with its CPU oracle
+    time, and 64^3 (Emin 1e-6, radius 4) at alpha 0 / 0.5 / 1 in serial and parallel load modes."""
+    import pgj
+    from pgj.density import filtered_noise, simp
+    from oracle import green, hex8, pcg
+
+    def timed(n, a, alpha, mode, reps):
+        res = pgj.Resident(n, load_mode=mode, batch=6 if mode == 'parallel' else None).capture()
+        res.material(torch.tensor(a, device='cuda'))
+        res.select(alpha)
+        times = []
+        for _ in range(reps + 1):
+            res.solutions.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            infos, checks = res.solve(rtol=1e-8)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        H = res.effective_tensor().cpu().numpy()
+        return {'solves_per_s': 1.0 / float(np.median(times[1:])), 'iterations': [i['iterations'] for i in infos],
+                'max_relative_residual': max(i['relative_residual'] for i in infos), 'host_checks': checks,
+                'CH_diag': np.diag(H).tolist()}, H
+
+    rows = {}
+    a16 = simp(filtered_noise(16, args.seed, 2), 1e-4)
+    rows['16^3 serial alpha 0.5'], H16 = timed(16, a16, 0.5, 'serial', 5)
+    C0 = hex8.isotropic(15 / 26, 5 / 13)
+    t0 = time.perf_counter()
+    solver = pcg.ResidentPCG(a16, C0, np.eye(6), batch=1, s=green.scale(a16, 0.5, C0))
+    infos, _ = solver.solve(rtol=1e-8)
+    cpu_s = time.perf_counter() - t0
+    rows['16^3 serial alpha 0.5']['cpu_oracle_solves_per_s'] = 1.0 / cpu_s
+    rows['16^3 serial alpha 0.5']['cpu_oracle_iterations'] = [i['iterations'] for i in infos]
+    a64 = simp(filtered_noise(64, args.seed, 4), 1e-6)
+    for alpha in (0.0, 0.5, 1.0):
+        for mode in ('serial', 'parallel'):
+            rows[f'64^3 {mode} alpha {alpha:g}'], _ = timed(64, a64, alpha, mode, 2)
+    out = {'metric': 'cold certified 6-load solves/s per configuration (rtol 1e-8)', 'unit': 'certified 6-load solves/s',
+           'n_gpus': 1, 'dtype': 'f64', 'data': 'synthetic (seeded filtered-noise densities)',
+           'higher_is_better': True, 'configs': rows}
+    print(json.dumps(out), flush=True)
+
+
 def design_steps_compare(args):
     """The default design-step workload (64^3, Emin 1e-6, warm starts, OC) with plain FP64 and mixed solves."""
     import pgj
@@ -528,6 +572,9 @@ def main():
     if args.form == 'hybrid':
         torch.cuda.set_device(0)
         return hybrid_form(args)
+    if args.form == 'configs':
+        torch.cuda.set_device(0)
+        return configs_form(args)
     if args.profile_stages:
         torch.cuda.set_device(0)
         return profile_stages(args)
