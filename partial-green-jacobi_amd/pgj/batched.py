@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._chunks import MaskPipeline
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -147,6 +148,7 @@ class BatchedDesigns:
         self.groups = {int(n): DesignGroup(n, d, loads, chunk, device=device) for n, d in sorted(sizes.items())}
         self.chunk = chunk
         self.init_graph = self.iter_graph = self.cert_graph = None
+        self._pipeline = None
 
     def _each(self, name):
         def run():
@@ -231,11 +233,9 @@ class BatchedDesigns:
         self._replay(self.cert_graph, 'certify')
         while True:
             self._replay(self.init_graph, 'initialize')
-            while True:
-                host_checks += 1
-                if not self._any_active():
-                    break
-                self._replay(self.iter_graph, 'iteration')
+            if self._pipeline is None:
+                self._pipeline = MaskPipeline([g.active for g in self.groups.values()])
+            host_checks += self._pipeline.run(lambda: self._replay(self.iter_graph, 'iteration'))
             self._replay(self.cert_graph, 'certify')
             diag = {n: torch.stack((g.norm, g.absolute, g.target2.sqrt(), g.iterations.to(torch.float64),
                                     g.bad.to(torch.float64)), dim=1).cpu().numpy() for n, g in self.groups.items()}

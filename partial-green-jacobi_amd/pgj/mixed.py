@@ -15,6 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._chunks import MaskPipeline
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -70,6 +71,7 @@ class MixedResident:
         self.active32 = view(self.state32, 5, torch.uint8, 1)
         self.iterations32 = view(self.state32, 4, torch.int64, 8)
         self.bad32 = view(self.state32, 6, torch.uint8, 1)
+        self._pipeline = None
         self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
         self.aos = f64(self.G, 3 * N)
         self.graphs = None
@@ -179,11 +181,9 @@ class MixedResident:
                 raise RuntimeError(('mixed-precision refinement did not certify', outer, v.tolist()))
             outer += 1
             self._run('begin')
-            while True:
-                host_checks += 1
-                if not bool(self.active32.any().item()):
-                    break
-                self._run('iteration')
+            if self._pipeline is None:
+                self._pipeline = MaskPipeline([self.active32])
+            host_checks += self._pipeline.run(lambda: self._run('iteration'))
             if bool(self.bad32.any().item()):
                 raise RuntimeError('FP32 inner PCG broke down (non-positive or non-finite curvature)')
             inner_total += self.iterations32.cpu().numpy()

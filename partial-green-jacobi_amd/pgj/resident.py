@@ -15,6 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._chunks import MaskPipeline
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -64,9 +65,7 @@ class Resident:
         self.iterations = self.state[o[4]:o[4] + 8 * B].view(torch.int64)
         self.active = self.state[o[5]:o[5] + B]
         self.bad = self.state[o[6]:o[6] + B]
-        # two pinned slots for the active masks of consecutive chunks (pipelined host checks)
-        self._flags = torch.zeros((2, B), dtype=torch.uint8, pin_memory=True)
-        self._flag_events = [torch.cuda.Event(), torch.cuda.Event()]
+        self._pipeline = MaskPipeline([self.active])
         self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, B), dtype=torch.uint8, device=dev)
         self.scale = torch.ones((1, 3 * N), dtype=torch.float64, device=dev)   # SoA (3, N)
         nz, ny, nx = shape
@@ -210,23 +209,7 @@ class Resident:
         self.cert_graph.replay()
         while True:
             self.init_graph.replay()
-            self._record_active(0)
-            host_checks += 1
-            if self._wait_active(0):
-                # Pipelined checks: the next chunk is enqueued before the mask of the previous one is read,
-                # so the device never waits for the host. A chunk enqueued after every load stopped is a no-op
-                # (inactive loads exit at once and keep their state), so results and iteration counts are those
-                # of the one-chunk-at-a-time loop.
-                self.iter_graph.replay()
-                self._record_active(1)
-                slot = 1
-                while True:
-                    self.iter_graph.replay()
-                    self._record_active(1 - slot)
-                    host_checks += 1
-                    if not self._wait_active(slot):
-                        break
-                    slot = 1 - slot
+            host_checks += self._pipeline.run(self.iter_graph.replay)
             self.cert_graph.replay()
             diag = torch.stack((self.norm, self.absolute, self.target2.sqrt(), self.iterations.to(torch.float64),
                                 self.bad.to(torch.float64)), dim=1).cpu().numpy()
@@ -240,14 +223,6 @@ class Resident:
         if not ok.all():
             raise RuntimeError(('true residual certification failed', info))
         return info, host_checks
-
-    def _record_active(self, slot):
-        self._flags[slot].copy_(self.active, non_blocking=True)
-        self._flag_events[slot].record()
-
-    def _wait_active(self, slot):
-        self._flag_events[slot].synchronize()
-        return bool(self._flags[slot].any())
 
     # -- responses ---------------------------------------------------------------------
     def energies(self, solutions=None, loads=None):
