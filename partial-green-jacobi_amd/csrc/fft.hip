@@ -705,6 +705,12 @@ extern "C" int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, con
   PGJ_REQUIRE((s == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   FFTArgs A = fft_args(*g, plan);
   PrecondLayout pl = precond_layout(*g, L);
+  const int kc = precond_kc(A.f, g->ny, L);
+  if (kc != A.f.kc) {
+    A.f.kc = kc;
+    A.f.nkb = (A.f.nkx + kc - 1) / kc;
+    A.f.lds_z = size_t(kc) * g->nz * 3 * 16 + 64 * 8;
+  }
   char* w = static_cast<char*>(ws);
   A.rin = r;
   A.s = s;

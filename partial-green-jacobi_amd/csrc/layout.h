@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstdlib>
 
+#include <algorithm>
+
 #include "../../include/pgj.h"
 
 namespace pgj {
@@ -234,13 +236,26 @@ struct PrecondLayout {
   size_t bytes;
 };
 
+// kx columns per block of the preconditioner apply's z pass: the geometry's
+// chunk, or narrower when the (load, ky, chunk) blocks would not cover 512
+// (few loads), which raises the block count of that pass.
+inline int precond_kc(const FFTGeom& f, int ny, int L) {
+  const int64_t rows = int64_t(L) * ny;
+  if (rows * f.nkb >= 512) return f.kc;
+  const int64_t want = std::min<int64_t>(f.nkx, (512 + rows - 1) / rows);
+  return std::max(1, int((f.nkx + want - 1) / want));
+}
+
 inline PrecondLayout precond_layout(const pgj_grid& g, int L, size_t cs = 16) {
   FFTGeom f = fft_geom(g);
   PrecondLayout p{};
   size_t off = 0;
+  // partial rows for the geometry's chunking and for the preconditioner apply's
+  const int kc = precond_kc(f, g.ny, L);
+  const int nkb = std::max(f.nkb, (f.nkx + kc - 1) / kc);
   p.spec = off; off = align256(off + size_t(L) * 3 * f.nf * cs);
   p.pr = off; off = align256(off + size_t(L) * g.nz * 6 * 8);
-  p.pg = off; off = align256(off + size_t(L) * f.nkb * g.ny * 4 * 8);
+  p.pg = off; off = align256(off + size_t(L) * nkb * g.ny * 4 * 8);
   p.bytes = off;
   return p;
 }
