@@ -681,7 +681,22 @@ extern "C" int pgj_plan_symbol(const pgj_grid* g, const void* plan, double* out,
   return PGJ_OK;
 }
 
-extern "C" size_t pgj_precond_ws_bytes(const pgj_grid* g, int L) { return precond_layout(*g, L).bytes; }
+// Cube grids with compile-time FFT kernels run the apply on SoA copies of r
+// and s (the kernels stream one component plane per block); the copies sit
+// after the preconditioner layout: L x 3N doubles (r, then z) and 3N (s).
+namespace {
+size_t precond_soa_offset(const pgj_grid& g, int L) { return precond_layout(g, L).bytes; }
+size_t precond_soa_bytes(const pgj_grid& g, int L) {
+  if (!static_fft(g)) return 0;
+  const size_t n3 = size_t(3) * g.nx * g.ny * g.nz;
+  return align256(size_t(L) * n3 * 8) + align256(n3 * 8);
+}
+}  // namespace
+
+extern "C" size_t pgj_precond_ws_bytes(const pgj_grid* g, int L) {
+  if (!g) return 0;
+  return precond_soa_offset(*g, L) + precond_soa_bytes(*g, L);
+}
 
 extern "C" int pgj_scale_spectrum(const pgj_grid* g, const void* plan, const double* s, void* s_hat, void* ws,
                                   void* stream) {
@@ -705,6 +720,26 @@ extern "C" int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, con
   PGJ_REQUIRE((s == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
   FFTArgs A = fft_args(*g, plan);
   PrecondLayout pl = precond_layout(*g, L);
+  if (static_fft(*g)) {
+    char* w = static_cast<char*>(ws);
+    double* rs = reinterpret_cast<double*>(w + precond_soa_offset(*g, L));
+    double* ss = reinterpret_cast<double*>(w + precond_soa_offset(*g, L) + align256(size_t(L) * 3 * A.N * 8));
+    hipStream_t st = as_stream(stream);
+    if (int e = pgj_aos_to_soa(g, L, r, rs, stream)) return e;
+    if (s)
+      if (int e = pgj_aos_to_soa(g, 1, s, ss, stream)) return e;
+    A.rin = rs;
+    A.s = s ? ss : nullptr;
+    A.shat = static_cast<const double2*>(s_hat);
+    A.spec = reinterpret_cast<double2*>(w + pl.spec);
+    A.pr = reinterpret_cast<double*>(w + pl.pr);
+    A.pg = reinterpret_cast<double*>(w + pl.pg);
+    A.zout = rs;  // r is read only by the forward pass
+    if (int e = launch_fxy_static(A, L, kFxyPre, st)) return e;
+    if (int e = launch_fz_static(A, L, kFzPre, st)) return e;
+    if (int e = launch_fxyi_static(A, L, st, kFxyiPre)) return e;
+    return pgj_soa_to_aos(g, L, rs, z, stream);
+  }
   const int kc = precond_kc(A.f, g->ny, L);
   if (kc != A.f.kc) {
     A.f.kc = kc;

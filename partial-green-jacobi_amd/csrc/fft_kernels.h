@@ -57,6 +57,6 @@ int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 bool static_fft(const pgj_grid& g);
 template <typename T> int launch_fxy_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
 template <typename T> int launch_fz_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
-template <typename T> int launch_fxyi_static(FFTArgsT<T>& A, int L, hipStream_t s);
+template <typename T> int launch_fxyi_static(FFTArgsT<T>& A, int L, hipStream_t s, int mode = kFxyiPCG);
 
 }  // namespace pgj

@@ -411,27 +411,29 @@ void fz_static(FFTArgsT<R> A) {
 }
 
 // --------------------------------------------------------------- inverse xy
-template <int N, typename R>
+// MODE kFxyiPre: the preconditioner apply alone (z = P(s w), no PCG state).
+template <int N, int MODE, typename R>
 __global__ __launch_bounds__(Cube<N>::TXY) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_xy<R>())))
 void fxyi_static(FFTArgsT<R> A) {
   using C = Cube<N>;
   using X = cplx<R>;
   constexpr int T = C::TXY;
+  constexpr bool PCG = MODE == kFxyiPCG;
   extern __shared__ __align__(16) unsigned char smem[];
   X* plane = reinterpret_cast<X*>(smem);
   X* tw = plane + N * C::RS;
   const int z = blockIdx.x, c = blockIdx.y, l = blockIdx.z;
   const int64_t NN = int64_t(N) * N * N;
-  if (threadIdx.x == 0 && z == 0 && c == 0) {
+  if (PCG && threadIdx.x == 0 && z == 0 && c == 0) {
     A.st.active[l] = A.st.active_next[l];
     A.st.it[l] = A.st.it_next[l];
     A.st.bad[l] = A.st.bad_next[l];
   }
   const int64_t off = (int64_t(l) * 3 + c) * NN + int64_t(z) * C::PLANE;
   // PCG: the displacement update u += alpha p of this iteration (alpha from the forward pass)
-  const bool move = A.p != nullptr && A.u != nullptr;
+  const bool move = PCG && A.p != nullptr && A.u != nullptr;
   const R ra = move ? R(A.st.alpha[l]) : R(0);
-  if (!A.st.active_next[l]) {
+  if (PCG && !A.st.active_next[l]) {
     // a load stopped by this iteration's test still takes its last displacement update
     if (move && A.st.stopped[l]) {
       const X* __restrict__ pp = reinterpret_cast<const X*>(A.p + off);
@@ -452,7 +454,7 @@ void fxyi_static(FFTArgsT<R> A) {
   // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
   const double* pgl = A.pg + int64_t(l) * A.f.nkb * N * 4;
   // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
-  const bool update = A.p != nullptr && !A.st.first[l];
+  const bool update = PCG && A.p != nullptr && !A.st.first[l];
   double mean = 0.0, beta = 0.0;
   if (A.shat || update) {
     // one pass: r.z (offset 0) and this component's s.w (offset 1 + c) partials
@@ -474,7 +476,7 @@ void fxyi_static(FFTArgsT<R> A) {
   constexpr bool EARLY = sizeof(R) == 4 && C::ITER == C::CH;
   const int64_t soff = (int64_t(design_of(l, A.group)) * 3 + c) * NN + int64_t(z) * C::PLANE;
   const X* __restrict__ sp = reinterpret_cast<const X*>(A.s + soff);
-  const X* __restrict__ pp = A.p != nullptr ? reinterpret_cast<const X*>(A.p + off) : nullptr;
+  const X* __restrict__ pp = PCG && A.p != nullptr ? reinterpret_cast<const X*>(A.p + off) : nullptr;
   X* __restrict__ up = move ? reinterpret_cast<X*>(A.u + off) : nullptr;
   X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
   const bool scaled = A.s != nullptr;
@@ -609,11 +611,16 @@ int fz_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
 }
 
 template <int N, typename R>
-int fxyi_static_n(FFTArgsT<R>& A, int L, hipStream_t s) {
+int fxyi_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   using C = Cube<N>;
   const size_t lds = C::lds_xy(sizeof(cplx<R>));
-  if (int e = allow_lds_static(fxyi_static<N, R>, lds)) return e;
-  hipLaunchKernelGGL((fxyi_static<N, R>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);
+  if (mode == kFxyiPre) {
+    if (int e = allow_lds_static(fxyi_static<N, kFxyiPre, R>, lds)) return e;
+    hipLaunchKernelGGL((fxyi_static<N, kFxyiPre, R>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);
+  } else {
+    if (int e = allow_lds_static(fxyi_static<N, kFxyiPCG, R>, lds)) return e;
+    hipLaunchKernelGGL((fxyi_static<N, kFxyiPCG, R>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);
+  }
   PGJ_LAUNCHED("fxyi_static");
   return PGJ_OK;
 }
@@ -647,17 +654,17 @@ int launch_fz_static(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
 }
 
 template <typename R>
-int launch_fxyi_static(FFTArgsT<R>& A, int L, hipStream_t s) {
-#define CALL(n) fxyi_static_n<n, R>(A, L, s)
+int launch_fxyi_static(FFTArgsT<R>& A, int L, hipStream_t s, int mode) {
+#define CALL(n) fxyi_static_n<n, R>(A, L, mode, s)
   PGJ_STATIC_SWITCH(CALL)
 #undef CALL
 }
 
 template int launch_fxy_static<double>(FFTArgsT<double>&, int, int, hipStream_t);
 template int launch_fz_static<double>(FFTArgsT<double>&, int, int, hipStream_t);
-template int launch_fxyi_static<double>(FFTArgsT<double>&, int, hipStream_t);
+template int launch_fxyi_static<double>(FFTArgsT<double>&, int, hipStream_t, int);
 template int launch_fxy_static<float>(FFTArgsT<float>&, int, int, hipStream_t);
 template int launch_fz_static<float>(FFTArgsT<float>&, int, int, hipStream_t);
-template int launch_fxyi_static<float>(FFTArgsT<float>&, int, hipStream_t);
+template int launch_fxyi_static<float>(FFTArgsT<float>&, int, hipStream_t, int);
 
 }  // namespace pgj
