@@ -104,6 +104,20 @@ def probe_stages(res, iters):
     return out, active
 
 
+def host_info():
+    """CPU model and logical CPU count of the host that timed the CPU baseline."""
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {'cpu_model': model, 'os_cpu_count': os.cpu_count()}
+
+
 def cpu_baseline(a, alpha, its_total, seconds):
     """Oracle (numpy FP64) resident PCG on a bounded sample of the same solve."""
     from threadpoolctl import threadpool_info
@@ -133,6 +147,7 @@ def cpu_baseline(a, alpha, its_total, seconds):
     per_solve = t_iter * its_total + 2 * 6 * t_cert
     threads = max([i.get('num_threads', 1) for i in threadpool_info()] or [1])
     return {'value': 1.0 / per_solve, 'unit': 'certified 6-load solves/s', 'cores': int(threads), 'kind': 'port',
+            'host': host_info(),
             'sample': f'{iters} iterations of the oracle resident PCG on load 3 (yz shear) of the first design '
                       f'step at {a.shape[0]}^3 (numpy FP64, pocketfft single-threaded, BLAS {threads} threads), '
                       f'{t_iter * 1e3:.1f} ms/iteration, extrapolated to the GPU step\'s {its_total} total '
@@ -397,6 +412,7 @@ def hybrid_cpu_baseline(phase, materials, orientation, total_iterations, seconds
     per_ch = per_node * n ** 3 * (total_iterations + 12)
     threads = max([i.get('num_threads', 1) for i in threadpool_info()] or [1])
     return {'value': 1.0 / per_ch, 'unit': 'C^H evaluations/s', 'cores': int(threads), 'kind': 'port',
+            'host': host_info(),
             'sample': f'{reps} oracle general-C21 applies on the first {planes} z-planes of the composite '
                       f'({per_node * 1e6:.2f} us per node), extrapolated to {n}^3 and the GPU run\'s '
                       f'{total_iterations} PCG iterations + 12 rhs/stress-equivalent applies; preconditioner '

@@ -380,7 +380,9 @@ int pgj_pcg32_post_step(int B, int64_t n, const float* r, float* z, const float*
  * solver/src/materials/material_field.py:14 PAIRS), read from
  * pool + 21*index[e] (index != NULL: shared phase rows and per-voxel rows,
  * solver/src/materials/hybrid_material.py:14-70) or pool + 21*e (index NULL,
- * the dense ablation). Only the grid's extents and voxel sizes are used. */
+ * the dense ablation). Only the grid's extents and voxel sizes are used.
+ * Index entries must lie in [0, pool rows): the kernels read the row they
+ * name without a bounds check (pgj.hybrid.HybridFIOperator checks once). */
 
 /* out_l = K(C) u_l for L AoS (L, 3N) loads; active (uint8[L], nullable)
  * zeroes inactive loads. Each block owns a 16 x 16 node column over a z-chunk
