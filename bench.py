@@ -300,6 +300,10 @@ def mixed_form(args):
 
 
 C21_ELEMENT_FLOPS = 660   # modal C21 element: 2 x 72 butterfly flops + 516 in the seven strain modes
+# HBM bytes of one one-load tiled C21 apply on the 64^3 composite, from the FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --form hybrid --profile-stages` scaled by the same 256 MiB copy calibration (read 2.0, write 1.0);
+# profiles/c21_tile_64cube_composite_traffic.md holds the commands and the figures.
+MEASURED_TRAFFIC_C21_64 = 38.36e6
 
 
 def composite(n, seed, radius):
@@ -333,6 +337,20 @@ def hybrid_form(args):
     pool, index, meta = build_hybrid_material(phase, materials, orientation=orientation)
     grid = (n, n, n)
     op = enable_fi_reference(HybridFIOperator(pool, index, grid, (1.0 / n,) * 3))
+    N = n ** 3
+    if args.profile_stages:
+        # counter-run workload: one 256 MiB device copy (calibration), then the one-load tiled apply
+        src = torch.ones(32 * 2 ** 20, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        u = torch.randn(1, 3 * N, dtype=torch.float64, device='cuda')
+        out = torch.empty_like(u)
+        for _ in range(args.probe_iters):
+            _lib.call('pgj_hybrid_apply', op.grid, 1, *op._pool(), _lib.ptr(u), None, _lib.ptr(out), _lib.stream())
+        torch.cuda.synchronize()
+        print(json.dumps({'launches': args.probe_iters, 'copy_bytes': 2 * src.numel() * 8,
+                          'pool_rows': meta['pool_rows']}), flush=True)
+        return
     for _ in range(max(args.warmup, 1)):
         C, info = pgj.compute_effective_stiffness_batched(op, tol=1e-8, max_iter=4000)
     torch.cuda.synchronize()
@@ -347,7 +365,6 @@ def hybrid_form(args):
     elapsed = time.perf_counter() - t_start
     iters = [info[0][J]['iters'] for J in range(6)]
     res = max(info[0][J]['rel_res'] for J in range(6))
-    N = n ** 3
 
     def kernel_us(name, L, reps=20):
         u = torch.randn(L, 3 * N, dtype=torch.float64, device='cuda')
@@ -383,7 +400,8 @@ def hybrid_form(args):
         'apply_us': {'tile_L1': t1, 'tile_L6': t6, 'gather_L1': g1},
         'apply_fp64_TFLOPs': {'L1': C21_ELEMENT_FLOPS * N / t1 / 1e6, 'L6': C21_ELEMENT_FLOPS * N * 6 / t6 / 1e6},
         'roofline': {'bound': 'hbm', 'kernel': 'c21_tile_kernel (one load)', 'achieved': achieved,
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': MEASURED_TRAFFIC_C21_64 if n == 64 and args.seed == 20261104 else None,
                      'flop_per_byte': C21_ELEMENT_FLOPS * N / bytes1},
         'cpu_baseline': cpu,
     }
