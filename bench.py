@@ -471,6 +471,10 @@ def configs_form(args):
     cpu_s = time.perf_counter() - t0
     rows['16^3 serial alpha 0.5']['cpu_oracle_solves_per_s'] = 1.0 / cpu_s
     rows['16^3 serial alpha 0.5']['cpu_oracle_iterations'] = [i['iterations'] for i in infos]
+    from oracle import design
+    H_ref = design.effective_tensor(a16, solver.solutions, C0)
+    rows['16^3 serial alpha 0.5']['ch_rel_err_vs_fp64_oracle'] = float(np.abs(H16 - H_ref).max() /
+                                                                      np.abs(H_ref).max())
     a64 = simp(filtered_noise(64, args.seed, 4), 1e-6)
     for alpha in (0.0, 0.5, 1.0):
         for mode in ('serial', 'parallel'):
