@@ -704,6 +704,18 @@ extern "C" int pgj_scale_spectrum(const pgj_grid* g, const void* plan, const dou
   PGJ_REQUIRE(plan && s && s_hat && ws, "invalid scale spectrum arguments");
   FFTArgs A = fft_args(*g, plan);
   PrecondLayout pl = precond_layout(*g, 1);
+  if (static_fft(*g)) {
+    // compile-time passes on an SoA copy of s (the first SoA slot of the workspace)
+    double* ss = reinterpret_cast<double*>(static_cast<char*>(ws) + precond_soa_offset(*g, 1));
+    if (int e = pgj_aos_to_soa(g, 1, s, ss, stream)) return e;
+    A.s = ss;
+    A.spec = static_cast<double2*>(s_hat);
+    A.pr = reinterpret_cast<double*>(static_cast<char*>(ws) + pl.pr);
+    A.pg = reinterpret_cast<double*>(static_cast<char*>(ws) + pl.pg);
+    hipStream_t st = as_stream(stream);
+    if (int e = launch_fxy_static(A, 1, kFxyScale, st)) return e;
+    return launch_fz_static(A, 1, kFzScale, st);
+  }
   A.s = s;
   A.spec = static_cast<double2*>(s_hat);
   A.pr = reinterpret_cast<double*>(static_cast<char*>(ws) + pl.pr);

@@ -64,7 +64,8 @@ def test_precond_properties_at_64():
 
 @pytest.mark.parametrize('n', [16, 24, 32, 48, 64, 96, 128])
 def test_static_kernels_match_runtime(n):
-    """The compile-time-shaped resident FFT kernels agree with the runtime-shaped ones (and the oracle to 48^3)."""
+    """The resident's FFT passes (SoA state) agree with the generic preconditioner apply on AoS vectors, which
+    runs the same compile-time passes on SoA copies; and with the oracle to 48^3 and at 128^3."""
     print(f'static vs runtime n={n}', flush=True)
     a = design.simp(design.filtered_noise(n, 5 + n, 2), 1e-4)
     res = pgj.Resident(n, load_mode='parallel', batch=3)
@@ -77,14 +78,10 @@ def test_static_kernels_match_runtime(n):
     res.r.copy_(res.to_soa(r))
     res.initialize()
     z = res.to_aos(res.p0).cpu().numpy()   # the first direction is M^-1 r
-    if n < 128:   # 128^3 planes exceed the runtime kernels' register budget: compile-time path only
-        shat_rt = res.op.scale_spectrum(s_aos)
-        assert rel(res.scale_hat.cpu().numpy().reshape(-1), shat_rt.cpu().numpy().reshape(-1)) < 1e-13
-        ref = res.op.green_precond(r, s_aos, shat_rt).cpu().numpy()
-        assert rel(z, ref) < 1e-12
-    else:
-        with pytest.raises(ValueError, match='compile-time'):
-            res.op.scale_spectrum(s_aos)
+    shat_rt = res.op.scale_spectrum(s_aos)
+    assert rel(res.scale_hat.cpu().numpy().reshape(-1), shat_rt.cpu().numpy().reshape(-1)) < 1e-13
+    ref = res.op.green_precond(r, s_aos, shat_rt).cpu().numpy()
+    assert rel(z, ref) < 1e-12
     if n <= 48 or n == 128:
         G6 = green.symbol(a.shape, hex8.voxel_of(a.shape), LAM, MU)
         assert rel(z, green.precond(r.cpu().numpy(), G6, a.shape, green.scale(a, 0.5, C0))) < 1e-12
@@ -124,3 +121,19 @@ def test_set_alpha_attaches_the_centred_log_diagonal():
     assert np.abs(op._log_diagonal.cpu().numpy() - q).max() < 1e-13
     s = np.exp(-0.25 * q)
     assert np.abs(op._scale.cpu().numpy() - s).max() < 1e-13
+
+
+@pytest.mark.parametrize('n', [16, 32, 48, 96, 128])
+def test_precond_static_cubes_match_oracle(n):
+    """Cube grids run the generic preconditioner apply through the compile-time FFT passes (SoA copies)."""
+    rng = np.random.default_rng(n)
+    a = np.exp(rng.uniform(-5, 0, (n, n, n)))
+    op = pgj.Operator(torch.tensor(a, device='cuda')).enable_green()
+    pgj.set_alpha(op, 0.5)
+    r = rng.normal(size=(2, 3 * n ** 3))
+    z = op.apply_precond(torch.tensor(r, device='cuda')).cpu().numpy()
+    grid = (n, n, n)
+    ref = green.precond(r, green.symbol(grid, hex8.voxel_of(grid), LAM, MU), grid, green.scale(a, 0.5, C0))
+    assert rel(z, ref) < 1e-12
+    one = op.apply_precond(torch.tensor(r[1:2], device='cuda')).cpu().numpy()
+    assert np.array_equal(one[0], z[1])   # the compile-time passes have one block shape for any load count
