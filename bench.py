@@ -78,29 +78,35 @@ def stage_bytes(L, n):
 
 
 def probe_stages(res, iters):
-    """Average device duration of each stage, HIP events on the launch stream."""
+    """Average device duration of each stage with all loads active: after one full iteration, each stage is
+    launched ``iters`` times back to back between two HIP events on the launch stream (a repeated stage keeps
+    every load active; the values it leaves are not used), so the launch gaps are amortised."""
     from pgj import _lib
     names = ['hex8_rowmarch', 'fxy', 'fz', 'fxyi']
     res._setup(range(res.batch), 1e-8)
     res.u.zero_()
     res.certify()
     res.initialize()
-    times = {k: [] for k in names}
-    bufs = [res.p0, res.p1]
-    for i in range(iters):
-        p, p_next = bufs[i % 2], bufs[(i + 1) % 2]
-        for stage, name in enumerate(names):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, 0, res.batch, _lib.ptr(res.op._plan),
-                      _lib.ptr(res.op._material_field), _lib.ptr(res.scale), _lib.ptr(res.scale_hat),
-                      _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(p), _lib.ptr(p_next), _lib.ptr(res.q),
-                      _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())
-            e1.record()
-            times[name].append((e0, e1))
-    torch.cuda.synchronize()
+
+    def launch(stage):
+        _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, 0, res.batch, _lib.ptr(res.op._plan),
+                  _lib.ptr(res.op._material_field), _lib.ptr(res.scale), _lib.ptr(res.scale_hat),
+                  _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(res.p0), _lib.ptr(res.p1), _lib.ptr(res.q),
+                  _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())
+
+    for stage in range(4):
+        launch(stage)
+    out = {}
+    for stage, name in enumerate(names):
+        launch(stage)                                   # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            launch(stage)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / iters
     active = res.active.cpu().numpy().tolist()
-    out = {k: float(np.mean([a.elapsed_time(b) for a, b in v[2:]])) for k, v in times.items()}
     return out, active
 
 

@@ -108,8 +108,13 @@ int make_slice(const pgj_grid* g, int L, int group, int l0, int nl, Slice* out) 
   PGJ_REQUIRE(L >= 1 && L <= kMaxResidentLoads && group >= 1 && L % group == 0,
               "resident set needs 1 <= L <= %d loads in designs of `group` loads, got L=%d group=%d",
               kMaxResidentLoads, L, group);
-  PGJ_REQUIRE(l0 >= 0 && nl >= 1 && l0 + nl <= L && l0 % group == 0 && nl % group == 0,
-              "load slice [%d, %d) must cover whole designs of %d loads within %d", l0, l0 + nl, group, L);
+  // whole designs, or loads of one design (the design offset of the slice is that design's, and every load
+  // inside the slice maps to it)
+  const bool whole = l0 % group == 0 && nl % group == 0;
+  const bool within = l0 >= 0 && nl >= 1 && l0 / group == (l0 + nl - 1) / group;
+  PGJ_REQUIRE(l0 >= 0 && nl >= 1 && l0 + nl <= L && (whole || within),
+              "load slice [%d, %d) must cover whole designs of %d loads, or lie in one, within %d", l0, l0 + nl,
+              group, L);
   out->L = L; out->group = group; out->l0 = l0; out->nl = nl;
   out->N = int64_t(g->nx) * g->ny * g->nz;
   out->nf = int64_t(g->nz) * g->ny * (g->nx / 2 + 1);

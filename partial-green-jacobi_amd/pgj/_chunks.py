@@ -1,4 +1,4 @@
-"""Pipelined host checks of the chunked resident solves.
+"""Host-side scheduling of the chunked resident solves: pipelined checks and two-stream halves.
 
 A solve replays a captured chunk of PCG iterations until every load's active
 mask is zero. ``MaskPipeline.run`` enqueues the next chunk before it reads the
@@ -45,3 +45,39 @@ class MaskPipeline:
             if not self._any(slot):
                 return checks
             slot = 1 - slot
+
+
+def halves(L, group):
+    """Two load slices that each cover whole designs or lie within one design, or None."""
+    h = L // 2
+    if L < 4 or L % 2:
+        return None
+    if h % group == 0 or L == group:
+        return (0, h), (h, L - h)
+    return None
+
+
+class TwoStreams:
+    """Runs ``launch(l0, nl)`` for two halves of a set, the second on a side stream forked from and joined
+    back to the current stream (also inside a captured graph), so the kernels of one half fill the tails of
+    the other's. The side stream has the default priority."""
+
+    def __init__(self, L, group, enable=True):
+        self.parts = halves(L, group) if enable else None
+        self.side = torch.cuda.Stream() if self.parts else None
+
+    def run(self, launch, L):
+        if self.parts is None:
+            launch(0, L)
+            return
+        (a0, an), (b0, bn) = self.parts
+        main = torch.cuda.current_stream()
+        fork = torch.cuda.Event()
+        fork.record(main)
+        self.side.wait_event(fork)
+        launch(a0, an)
+        with torch.cuda.stream(self.side):
+            launch(b0, bn)
+        join = torch.cuda.Event()
+        join.record(self.side)
+        main.wait_event(join)

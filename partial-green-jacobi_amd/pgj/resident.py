@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._chunks import MaskPipeline
+from ._chunks import MaskPipeline, TwoStreams
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -66,6 +66,7 @@ class Resident:
         self.active = self.state[o[5]:o[5] + B]
         self.bad = self.state[o[6]:o[6] + B]
         self._pipeline = MaskPipeline([self.active])
+        self._halves = TwoStreams(self.batch, self.batch)
         self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, B), dtype=torch.uint8, device=dev)
         self.scale = torch.ones((1, 3 * N), dtype=torch.float64, device=dev)   # SoA (3, N)
         nz, ny, nx = shape
@@ -128,13 +129,21 @@ class Resident:
                   _lib.ptr(self.scale_hat), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.state),
                   _lib.ptr(self.ws), _lib.stream())
 
-    def iteration(self, count=None):
-        """``count`` fused masked PCG iterations (``resident.py:125-131``)."""
-        _lib.call('pgj_resident_iterate', self.op.grid, self.batch, self.batch, 0, self.batch, _lib.ptr(self.op._plan),
+    def _iterate_slice(self, l0, nl, count):
+        _lib.call('pgj_resident_iterate', self.op.grid, self.batch, self.batch, l0, nl, _lib.ptr(self.op._plan),
                   _lib.ptr(self.op._material_field), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
                   _lib.ptr(self.u), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
-                  _lib.ptr(self.state), _lib.ptr(self.ws), int(count or self.chunk),
-                  _lib.stream())
+                  _lib.ptr(self.state), _lib.ptr(self.ws), count, _lib.stream())
+
+    def iteration(self, count=None):
+        """``count`` fused masked PCG iterations (``resident.py:125-131``).
+
+        In the parallel load mode with an even group of four or more loads, the two halves of the group run on
+        two streams (forked from and joined to the current one, also inside a captured graph), so the kernels of
+        one half fill the tails of the other's. Every load's arithmetic is independent of the other loads, so
+        the results are those of one launch over the whole group."""
+        count = int(count or self.chunk)
+        self._halves.run(lambda l0, nl: self._iterate_slice(l0, nl, count), self.batch)
 
     def capture(self):
         """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``)."""
