@@ -45,6 +45,8 @@ def parse():
     p.add_argument('--move', type=float, default=0.2)
     p.add_argument('--rmin', type=float, default=3.0)
     p.add_argument('--probe-iters', type=int, default=10)
+    p.add_argument('--streams', type=int, default=2,
+                   help='streams the parallel load group is split over in the solve (slices of batch/streams loads)')
     p.add_argument('--cpu-seconds', type=float, default=15.0)
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid', 'configs'), default='plain',
@@ -642,7 +644,7 @@ def main():
     n = args.n
     rho0 = filtered_noise(n, args.seed, args.radius)
     res = pgj.Resident(n, np.eye(6), 'parallel' if args.batch > 1 else 'serial',
-                       batch=args.batch if args.batch > 1 else None).capture()
+                       batch=args.batch if args.batch > 1 else None, streams=args.streams).capture()
     outer = PaperOuter(n, args.volume, rmin=args.rmin, beta=4.0, eta=0.5, emin=args.emin, move=args.move).capture()
     outer.set_initial(torch.tensor(rho0, device='cuda'))
     log = []
@@ -700,7 +702,8 @@ def main():
         'dtype': 'f64', 'data': 'synthetic (seeded filtered-noise density)',
         'config': {'workload': f'{n}^3 periodic Q1/HEX8 SIMP (E0=1, nu=0.3), 6 unit-strain loads, Emin={args.emin:g}, '
                                f'filtered-noise density r={args.radius} seed {args.seed}, alpha={args.alpha:g}, '
-                               f'parallel load mode batch {res.batch}, one Study-II OC design step per step '
+                               f'parallel load mode batch {res.batch} over {args.streams} streams, one Study-II OC '
+                               f'design step per step '
                                f'(volume {args.volume:g}, move {args.move:g}, rmin {args.rmin:g}) with warm starts '
                                f'on device', 'n': n, 'loads': 6, 'alpha': args.alpha, 'emin': args.emin,
                    'batch': res.batch, 'rtol': 1e-8},

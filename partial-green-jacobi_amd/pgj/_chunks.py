@@ -47,37 +47,39 @@ class MaskPipeline:
             slot = 1 - slot
 
 
-def halves(L, group):
-    """Two load slices that each cover whole designs or lie within one design, or None."""
-    h = L // 2
-    if L < 4 or L % 2:
+def split_loads(L, group, parts):
+    """``parts`` equal load slices that each cover whole designs or lie within one design, or None."""
+    if parts < 2 or L % parts or L // parts < 1:
         return None
-    if h % group == 0 or L == group:
-        return (0, h), (h, L - h)
+    k = L // parts
+    if k % group == 0 or (L == group):
+        return [(i * k, k) for i in range(parts)]
     return None
 
 
-class TwoStreams:
-    """Runs ``launch(l0, nl)`` for two halves of a set, the second on a side stream forked from and joined
-    back to the current stream (also inside a captured graph), so the kernels of one half fill the tails of
-    the other's. The side stream has the default priority."""
+class StreamSplit:
+    """Runs ``launch(l0, nl)`` for ``parts`` slices of a set, the first on the current stream and the others on
+    side streams forked from and joined back to it (also inside a captured graph), so the kernels of one slice
+    fill the tails of the others'. Side streams have the default priority."""
 
-    def __init__(self, L, group, enable=True):
-        self.parts = halves(L, group) if enable else None
-        self.side = torch.cuda.Stream() if self.parts else None
+    def __init__(self, L, group, parts=2, enable=True):
+        self.parts = split_loads(L, group, parts) if enable else None
+        self.side = [torch.cuda.Stream() for _ in range(len(self.parts) - 1)] if self.parts else []
 
     def run(self, launch, L):
         if self.parts is None:
             launch(0, L)
             return
-        (a0, an), (b0, bn) = self.parts
         main = torch.cuda.current_stream()
         fork = torch.cuda.Event()
         fork.record(main)
-        self.side.wait_event(fork)
-        launch(a0, an)
-        with torch.cuda.stream(self.side):
-            launch(b0, bn)
-        join = torch.cuda.Event()
-        join.record(self.side)
-        main.wait_event(join)
+        for st in self.side:
+            st.wait_event(fork)
+        l0, nl = self.parts[0]
+        launch(l0, nl)
+        for st, (l0, nl) in zip(self.side, self.parts[1:]):
+            with torch.cuda.stream(st):
+                launch(l0, nl)
+            join = torch.cuda.Event()
+            join.record(st)
+            main.wait_event(join)

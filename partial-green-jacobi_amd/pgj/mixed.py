@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._chunks import MaskPipeline, TwoStreams
+from ._chunks import MaskPipeline, StreamSplit
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -73,7 +73,7 @@ class MixedResident:
         self.bad32 = view(self.state32, 6, torch.uint8, 1)
         self._pipeline = None
         # two streams pay off for the FP32 passes from 96^3 (64^3: 10.0 against 9.7 ms per cold solve)
-        self._halves = TwoStreams(self.L, self.G, enable=self.n >= 96)
+        self._halves = StreamSplit(self.L, self.G, 2, enable=self.n >= 96)
         self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
         self.aos = f64(self.G, 3 * N)
         self.graphs = None

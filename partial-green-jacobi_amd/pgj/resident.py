@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._chunks import MaskPipeline, TwoStreams
+from ._chunks import MaskPipeline, StreamSplit
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -30,7 +30,7 @@ class Resident:
     """
 
     def __init__(self, n, loads=LOADS6, load_mode='serial', batch=None, chunk=4, lam=LAMBDA_STUDY2,
-                 mu=MU_STUDY2, device='cuda'):
+                 mu=MU_STUDY2, device='cuda', streams=2):
         if load_mode not in ('serial', 'parallel'):
             raise ValueError('load_mode must be serial or parallel')
         if chunk < 2 or chunk % 2:
@@ -66,7 +66,7 @@ class Resident:
         self.active = self.state[o[5]:o[5] + B]
         self.bad = self.state[o[6]:o[6] + B]
         self._pipeline = MaskPipeline([self.active])
-        self._halves = TwoStreams(self.batch, self.batch)
+        self._halves = StreamSplit(self.batch, self.batch, streams)
         self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, B), dtype=torch.uint8, device=dev)
         self.scale = torch.ones((1, 3 * N), dtype=torch.float64, device=dev)   # SoA (3, N)
         nz, ny, nx = shape
