@@ -149,11 +149,33 @@ class BatchedDesigns:
         self.chunk = chunk
         self.init_graph = self.iter_graph = self.cert_graph = None
         self._pipeline = None
+        self._side = []   # side streams (default priority) of the concurrent iteration
 
     def _each(self, name):
         def run():
             for g in self.groups.values():
                 getattr(g, name)()
+        return run
+
+    def _concurrent(self, name):
+        """The sets' launches on one stream each, forked from and joined to the current stream (the sets share
+        no buffers); captured into the iteration graph, so small sets fill the CUs a large set leaves idle."""
+        groups = list(self.groups.values())
+        while len(self._side) < len(groups) - 1:
+            self._side.append(torch.cuda.Stream())
+
+        def run():
+            main = torch.cuda.current_stream()
+            fork = torch.cuda.Event()
+            fork.record(main)
+            getattr(groups[0], name)()
+            for g, st in zip(groups[1:], self._side):
+                st.wait_event(fork)
+                with torch.cuda.stream(st):
+                    getattr(g, name)()
+                join = torch.cuda.Event()
+                join.record(st)
+                main.wait_event(join)
         return run
 
     def tune(self, reps=6, candidates=None):
@@ -201,7 +223,7 @@ class BatchedDesigns:
         self._each('initialize')()
         self._each('iteration')()
         torch.cuda.synchronize()
-        for name, fn in (('init_graph', self._each('initialize')), ('iter_graph', self._each('iteration')),
+        for name, fn in (('init_graph', self._each('initialize')), ('iter_graph', self._concurrent('iteration')),
                          ('cert_graph', self._each('certify'))):
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph):
