@@ -40,6 +40,8 @@ struct FFTArgsT {
   StateView st;
   int soa;               // vectors (and s) are SoA (L, 3, N) instead of AoS (L, N, 3)
   int group;             // loads per design (s is (designs, 3N), s_hat (designs, 3, nf)); 0: shared
+  int zwide;             // compile-time z pass of a cube on its wide kx chunks (zpass_wide); the same for
+                         // every launch that writes or reads one workspace's z partials
 };
 using FFTArgs = FFTArgsT<double>;
 
@@ -55,6 +57,10 @@ int launch_fz_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 // Compile-time-shaped SoA kernels for cubic grids (n in 16, 24, 32, 48, 64, 96, 128).
 bool static_fft(const pgj_grid& g);
+// Whether the z pass of a solve with L loads in all (its workspace's load count) and elements of es bytes
+// takes the wide kx chunks: only where the default chunks leave a partial second round of blocks on the
+// device and the wide ones fit in one (64^3 FP64 at 6 loads on 256 CUs).
+bool zpass_wide(const pgj_grid& g, int L, int es);
 template <typename T> int launch_fxy_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
 template <typename T> int launch_fz_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
 template <typename T> int launch_fxyi_static(FFTArgsT<T>& A, int L, hipStream_t s, int mode = kFxyiPCG);

@@ -46,11 +46,13 @@ struct Cube {
 // (c, kz). A last chunk narrower than KC is supported (columns past NKX are
 // zero in LDS and skipped). 128^3 takes 13 columns in FP32 (104-byte runs
 // instead of 40) but keeps 5 in FP64, where 13 would need 147 VGPRs at 2 blocks
-// per CU; 17 columns at 96^3 measured no faster than 7.
-template <int N, typename R>
+// per CU; 17 columns at 96^3 measured no faster than 7. KCX > 0 overrides the
+// chunk width (the wide 64^3 FP64 instance, kWideKC).
+template <int N, typename R, int KCX = 0>
 struct ZPass {
   static constexpr int NKX = N / 2 + 1;
-  static constexpr int KC = N == 16    ? 9
+  static constexpr int KC = KCX > 0    ? KCX
+                            : N == 16  ? 9
                             : N == 24  ? 13
                             : N == 32  ? 17
                             : N == 48  ? 5
@@ -64,8 +66,23 @@ struct ZPass {
   // the columns are written back (2 blocks of the 128^3 FP32 pass fit one CU)
   static constexpr size_t lds = (size_t(3) * N * KC + N) * 2 * sizeof(R);
   static constexpr int occ = lds_occupancy(lds, TZ);
+  static constexpr int per_cu = occ * 4 / (TZ / 64) > 0 ? occ * 4 / (TZ / 64) : 1;  // resident blocks per CU
   using ZCols = Lines<3 * KC, KC, N * KC, 1, KC, true>;  // z transforms on (component, column)
 };
+
+// Wide z chunks: at 64^3 in FP64 the default 11 columns give 3 chunks per ky row
+// in 34 KiB of LDS (4 blocks per CU), 17 columns give 2 chunks in 52 KiB (3 per
+// CU). A 6-load launch is 1152 blocks against 1024 slots of 256 CUs with the
+// default, a partial second round, and 768 blocks in exactly one round with the
+// wide chunks: the pass takes 31 us instead of 39 us. zpass_wide decides.
+template <int N, typename R>
+constexpr int kWideKC = N == 64 && sizeof(R) == 8 ? 17 : 0;
+
+// Chunks per ky row of the z pass, which lays out its per-block partials.
+template <int N, typename R>
+__host__ __device__ constexpr int z_chunks(bool wide) {
+  return kWideKC<N, R> > 0 && wide ? ZPass<N, R, (kWideKC<N, R> > 0 ? kWideKC<N, R> : 1)>::NKB : ZPass<N, R>::NKB;
+}
 
 // Global -> LDS copy of COUNT values in two phases, so that independent work
 // can run while the loads are in flight: load() issues every load of a thread
@@ -146,7 +163,7 @@ void fxy_static(FFTArgsT<R> A) {
     // PCG: the step length is reduced while the r and q loads are in flight
     if (MODE == kFxyPCG && i0 == 0) {
       double pq, rz;
-      constexpr int NKB = ZPass<N, R>::NKB;
+      const int NKB = z_chunks<N, R>(A.zwide);
       // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
       wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * A.f.nkb * N * 4,
                              NKB * N, 4, pq, rz);
@@ -215,12 +232,12 @@ void fxy_static(FFTArgsT<R> A) {
 }
 
 // ---------------------------------------------------------------- z + Green
-template <int N, int MODE, typename R>
-__global__ __launch_bounds__((ZPass<N, R>::TZ)) __attribute__((amdgpu_waves_per_eu((ZPass<N, R>::occ))))
+template <int N, int MODE, typename R, int KCX = 0>
+__global__ __launch_bounds__((ZPass<N, R, KCX>::TZ)) __attribute__((amdgpu_waves_per_eu((ZPass<N, R, KCX>::occ))))
 void fz_static(FFTArgsT<R> A) {
   using C = Cube<N>;
   using X = cplx<R>;
-  using Z = ZPass<N, R>;
+  using Z = ZPass<N, R, KCX>;
   constexpr int T = Z::TZ, KC = Z::KC, NKX = C::NKX;
   extern __shared__ __align__(16) unsigned char smem[];
   X* cols = reinterpret_cast<X*>(smem);
@@ -450,7 +467,7 @@ void fxyi_static(FFTArgsT<R> A) {
   StageIn<N, T, X> twl;
   in.load(A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX, [](int e) { return e; });
   twl.load(A.twy, [](int e) { return e; });
-  constexpr int NKB = ZPass<N, R>::NKB;
+  const int NKB = z_chunks<N, R>(A.zwide);
   // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
   const double* pgl = A.pg + int64_t(l) * A.f.nkb * N * 4;
   // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
@@ -581,33 +598,62 @@ int fxy_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   return PGJ_OK;
 }
 
-template <int N, typename R>
-int fz_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
-  using C = Cube<N>;
-  using Z = ZPass<N, R>;
+template <int N, typename R, int KCX>
+int fz_static_kc(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
+  using Z = ZPass<N, R, KCX>;
   const size_t lds = Z::lds;
   // the per-block partials are laid out for the geometry's chunk count
   PGJ_REQUIRE(Z::NKB <= A.f.nkb, "z pass chunks (%d) exceed the partial layout (%d)", Z::NKB, A.f.nkb);
   dim3 grid(N, Z::NKB, L), block(Z::TZ);
   switch (mode) {
     case kFzScale:
-      if (int e = allow_lds_static(fz_static<N, kFzScale, R>, lds)) return e;
-      hipLaunchKernelGGL((fz_static<N, kFzScale, R>), grid, block, lds, s, A);
+      if (int e = allow_lds_static(fz_static<N, kFzScale, R, KCX>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzScale, R, KCX>), grid, block, lds, s, A);
       break;
     case kFzPre:
-      if (int e = allow_lds_static(fz_static<N, kFzPre, R>, lds)) return e;
-      hipLaunchKernelGGL((fz_static<N, kFzPre, R>), grid, block, lds, s, A);
+      if (int e = allow_lds_static(fz_static<N, kFzPre, R, KCX>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzPre, R, KCX>), grid, block, lds, s, A);
       break;
     case kFzInit:
-      if (int e = allow_lds_static(fz_static<N, kFzInit, R>, lds)) return e;
-      hipLaunchKernelGGL((fz_static<N, kFzInit, R>), grid, block, lds, s, A);
+      if (int e = allow_lds_static(fz_static<N, kFzInit, R, KCX>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzInit, R, KCX>), grid, block, lds, s, A);
       break;
     default:
-      if (int e = allow_lds_static(fz_static<N, kFzPCG, R>, lds)) return e;
-      hipLaunchKernelGGL((fz_static<N, kFzPCG, R>), grid, block, lds, s, A);
+      if (int e = allow_lds_static(fz_static<N, kFzPCG, R, KCX>, lds)) return e;
+      hipLaunchKernelGGL((fz_static<N, kFzPCG, R, KCX>), grid, block, lds, s, A);
   }
   PGJ_LAUNCHED("fz_static");
   return PGJ_OK;
+}
+
+template <int N, typename R>
+int fz_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
+  if constexpr (kWideKC<N, R> > 0) {
+    if (A.zwide) return fz_static_kc<N, R, kWideKC<N, R>>(A, L, mode, s);
+  }
+  return fz_static_kc<N, R, 0>(A, L, mode, s);
+}
+
+// CUs of the current device, cached per device; 0 if the query fails.
+static int device_cus() {
+  static int cache[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cache[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    cache[dev] = cus;
+  }
+  return cache[dev];
+}
+
+bool zpass_wide(const pgj_grid& g, int L, int es) {
+  if (es != 8 || !static_cube(g) || g.nx != 64 || L < 1) return false;
+  const int64_t cus = device_cus();
+  using Zn = ZPass<64, double>;
+  using Zw = ZPass<64, double, kWideKC<64, double>>;
+  const int64_t narrow = int64_t(Zn::NKB) * 64 * L, wide = int64_t(Zw::NKB) * 64 * L;
+  return cus > 0 && narrow > cus * Zn::per_cu && wide <= cus * Zw::per_cu;
 }
 
 template <int N, typename R>

@@ -91,6 +91,27 @@ def test_resident_64_certifies():
     assert np.linalg.eigvalsh(H).min() > 0
 
 
+def test_resident_64_z_chunk_widths_agree():
+    """A 6-load 64^3 FP64 workspace runs the z pass on 17-column kx chunks where 6 loads then fill one round of
+    blocks (256 CUs); a 1-load workspace keeps 11-column chunks. One 6-load launch per stage, the two 3-load
+    halves and six 1-load solves certify with the same iteration counts (+-2: the r.z partials regroup) and C^H."""
+    n = 64
+    a = torch.tensor(density(n, 1e-6, 4), device='cuda')
+    runs = []
+    for kw in (dict(load_mode='parallel', batch=6, streams=1), dict(load_mode='parallel', batch=6, streams=2),
+               dict(load_mode='serial')):
+        r = pgj.Resident(n, **kw).capture()
+        r.material(a)
+        r.select('0.5')
+        infos, _ = r.solve()
+        assert all(i['converged'] and i['relative_residual'] <= 1e-8 for i in infos), kw
+        runs.append((np.array([i['iterations'] for i in infos]), r.effective_tensor().cpu().numpy()))
+    its, H = runs[2]
+    for other_its, other_H in runs[:2]:
+        assert np.abs(other_its - its).max() <= 2, (other_its, its)
+        assert np.abs(other_H - H).max() <= 1e-9 * np.abs(H).max()
+
+
 def test_resident_two_x_per_lane_matches_oracle():
     """nx = 96 (two x per lane in the row-march PCG kernel, runtime FFT) vs the oracle recurrence."""
     shape = (6, 8, 96)
