@@ -58,8 +58,8 @@ int launch_fxyi_mode(FFTArgs& A, int L, int mode, hipStream_t s);
 // Compile-time-shaped SoA kernels for cubic grids (n in 16, 24, 32, 48, 64, 96, 128).
 bool static_fft(const pgj_grid& g);
 // Whether the z pass of a solve with L loads in all (its workspace's load count) and elements of es bytes
-// takes the wide kx chunks: only where the default chunks leave a partial second round of blocks on the
-// device and the wide ones fit in one (64^3 FP64 at 6 loads on 256 CUs).
+// takes the wide kx chunks: where the default chunks need more than one round of blocks on the device
+// (64^3 FP64 from 6 loads on 256 CUs).
 bool zpass_wide(const pgj_grid& g, int L, int es);
 template <typename T> int launch_fxy_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
 template <typename T> int launch_fz_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);

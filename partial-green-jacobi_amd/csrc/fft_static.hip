@@ -72,9 +72,13 @@ struct ZPass {
 
 // Wide z chunks: at 64^3 in FP64 the default 11 columns give 3 chunks per ky row
 // in 34 KiB of LDS (4 blocks per CU), 17 columns give 2 chunks in 52 KiB (3 per
-// CU). A 6-load launch is 1152 blocks against 1024 slots of 256 CUs with the
-// default, a partial second round, and 768 blocks in exactly one round with the
-// wide chunks: the pass takes 31 us instead of 39 us. zpass_wide decides.
+// CU), and the symbol and s_hat are read by 2 blocks per ky row instead of 3. A
+// 6-load launch is 1152 blocks against 1024 slots of 256 CUs with the default, a
+// partial second round, and 768 blocks in exactly one round with the wide
+// chunks: the pass takes 31 us instead of 39 us. Launches of many rounds (a
+// batched set of 72 loads) gain from the fewer re-reads; launches that fit one
+// round with the default (up to 5 loads) keep it, since a wide block does more
+// work in that single round. zpass_wide decides.
 template <int N, typename R>
 constexpr int kWideKC = N == 64 && sizeof(R) == 8 ? 17 : 0;
 
@@ -651,9 +655,8 @@ bool zpass_wide(const pgj_grid& g, int L, int es) {
   if (es != 8 || !static_cube(g) || g.nx != 64 || L < 1) return false;
   const int64_t cus = device_cus();
   using Zn = ZPass<64, double>;
-  using Zw = ZPass<64, double, kWideKC<64, double>>;
-  const int64_t narrow = int64_t(Zn::NKB) * 64 * L, wide = int64_t(Zw::NKB) * 64 * L;
-  return cus > 0 && narrow > cus * Zn::per_cu && wide <= cus * Zw::per_cu;
+  const int64_t narrow = int64_t(Zn::NKB) * 64 * L;  // blocks of the default pass
+  return cus > 0 && narrow > cus * Zn::per_cu;
 }
 
 template <int N, typename R>

@@ -92,8 +92,8 @@ def test_resident_64_certifies():
 
 
 def test_resident_64_z_chunk_widths_agree():
-    """A 6-load 64^3 FP64 workspace runs the z pass on 17-column kx chunks where 6 loads then fill one round of
-    blocks (256 CUs); a 1-load workspace keeps 11-column chunks. One 6-load launch per stage, the two 3-load
+    """A 6-load 64^3 FP64 workspace runs the z pass on 17-column kx chunks (its 11-column blocks would need more
+    than one round on 256 CUs); a 1-load workspace keeps 11-column chunks. One 6-load launch per stage, the two 3-load
     halves and six 1-load solves certify with the same iteration counts (+-2: the r.z partials regroup) and C^H."""
     n = 64
     a = torch.tensor(density(n, 1e-6, 4), device='cuda')
