@@ -201,9 +201,13 @@ int pgj_alpha_select(const pgj_grid* g, int method, const double* params, const 
  * design, the reference's case, and several designs of one grid in one
  * launch is the batched form. Vectors, `state` and `ws` are laid out for all
  * L loads; each call below works on the slice of loads [l0, l0 + nl), which
- * must cover whole designs, and leaves every other load untouched, so designs
- * can join and leave a running set (continuous batching). Pass l0 = 0,
- * nl = L for the whole set. All pointers are to the whole set's buffers. */
+ * covers whole designs or lies within one design, and leaves every other load
+ * untouched, so designs can join and leave a running set (continuous
+ * batching) and the loads of one design can run as slices on several
+ * streams. Pass l0 = 0, nl = L for the whole set. All pointers are to the
+ * whole set's buffers, and every call on a set passes the set's L: the FFT
+ * passes' block shapes, and so the layout of the partial sums one pass leaves
+ * for the next, follow it. */
 
 /* rhs_l for the L macro strains (host L*6), norms, targets
  * max(rtol |f|, 1e-13/sqrt(nx))^2, iterations = 0, bad = 0.
