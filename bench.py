@@ -605,7 +605,7 @@ def serve_form(args):
         'unserved': unserved, 'max_relative_residual': rel, 'slots': slots, 'resident_gib': resident,
         'server_phases_s': phases,
         'plain_sequential': {'value': len(base_done) / base.elapsed, 'completed': len(base_done),
-                             'unserved': base.unserved, 'p50_ms': 1e3 * base_stats['p50'],
+                             'unserved': base.unserved, 'failed_certification': base.failed, 'p50_ms': 1e3 * base_stats['p50'],
                              'p99_ms': 1e3 * base_stats['p99']},
         'speedup_vs_plain_sequential': (len(done) / elapsed) / max(len(base_done) / base.elapsed, 1e-30),
     }

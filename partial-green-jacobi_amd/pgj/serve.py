@@ -357,6 +357,7 @@ class SequentialServer:
         self.workers = {}
         self.warm = {}
         self.unserved = 0
+        self.failed = []   # (client id, grid, Emin, alpha) of requests whose solve failed certification
 
     def _worker(self, n, emin):
         w = self.workers.get((n, emin))
@@ -400,7 +401,15 @@ class SequentialServer:
                 time.sleep(wait)
             req = Request(clients[cid], at)
             req.start = now()
-            infos, H, count = self._step(clients[cid])
+            try:
+                infos, H, count = self._step(clients[cid])
+            except RuntimeError:
+                # the plain path hit the 4000-iteration cap short of the tolerance (the reference raises
+                # there too): the request fails, the client keeps its design, the queue goes on
+                torch.cuda.synchronize()
+                c = clients[cid]
+                self.failed.append((cid, c.n, c.emin, c.alpha))
+                continue
             torch.cuda.synchronize()
             req.end = now()
             clients[cid].done += 1
