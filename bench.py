@@ -172,7 +172,7 @@ def profile_stages(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     n = args.n
-    res = pgj.Resident(n, np.eye(6), 'parallel', batch=args.batch).capture()
+    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=args.batch).capture()
     res.material(torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda'))
     res.select(args.alpha)
     durations, active = probe_stages(res, args.probe_iters)
@@ -219,7 +219,7 @@ def batched_form(args):
     # the plain drop-in on the same designs, one at a time (parallel load mode, one group of 6)
     solo_time, solo_its, ch_err = 0.0, 0, 0.0
     for n in sizes:
-        res = pgj.Resident(n, np.eye(6), 'parallel', batch=6).capture()
+        res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=6).capture()
         for d, (a, alpha) in enumerate(designs[n]):
             res.solutions.zero_()
             res.material(torch.tensor(a, device='cuda'))
@@ -259,7 +259,7 @@ def mixed_form(args):
     rows = {}
     for n, emin, radius in cases:
         a = torch.tensor(simp(filtered_noise(n, args.seed, radius), emin), device='cuda')
-        plain = pgj.Resident(n, np.eye(6), 'parallel', batch=6).capture()
+        plain = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=6).capture()
         plain.material(a)
         plain.select(args.alpha)
         mixed = MixedResident(n).capture()
@@ -520,7 +520,7 @@ def design_steps_compare(args):
                            move=args.move).capture()
         outer.set_initial(rho0)
         if kind == 'fp64':
-            res = pgj.Resident(n, np.eye(6), 'parallel', batch=6).capture()
+            res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=6).capture()
         else:
             res = MixedResident(n).capture()
         lat, its, chs = [], [], []
@@ -532,7 +532,7 @@ def design_steps_compare(args):
                 res.material(outer.a)
                 res.select(args.alpha)
                 infos, _ = res.solve(rtol=1e-8)
-                _, local, _ = res.energy('shear')
+                _, local, _ = res.objective_energy('shear')
                 H = res.effective_tensor()
             else:
                 res.material(0, outer.a)
@@ -643,7 +643,7 @@ def main():
     torch.cuda.set_device(0)
     n = args.n
     rho0 = filtered_noise(n, args.seed, args.radius)
-    res = pgj.Resident(n, np.eye(6), 'parallel' if args.batch > 1 else 'serial',
+    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel' if args.batch > 1 else 'serial',
                        batch=args.batch if args.batch > 1 else None, streams=args.streams).capture()
     outer = PaperOuter(n, args.volume, rmin=args.rmin, beta=4.0, eta=0.5, emin=args.emin, move=args.move).capture()
     outer.set_initial(torch.tensor(rho0, device='cuda'))
@@ -654,7 +654,7 @@ def main():
         res.material(outer.a)
         res.select(args.alpha)
         infos, checks = res.solve(rtol=1e-8)
-        obj, local, _ = res.energy('shear')
+        obj, local, _ = res.objective_energy('shear')
         H = res.effective_tensor()
         count = outer.step(local)
         outer.advance()

@@ -210,15 +210,17 @@ int pgj_alpha_select(const pgj_grid* g, int method, const double* params, const 
  * for the next, follow it. */
 
 /* rhs_l for the L macro strains (host L*6), norms, targets
- * max(rtol |f|, 1e-13/sqrt(nx))^2, iterations = 0, bad = 0.
+ * max(rtol |f|, 1e-13/sqrt(nx))^2 (NaN if |f| is NaN, as torch.maximum),
+ * iterations = 0, bad = 0, and the per-load iteration cap `cap` (>= 1; the
+ * reference's is 4000, exp/asg_gpu_resident_20261103/resident.py:50).
  * Replaces Resident._solve_batch set-up (resident.py:186-191). */
 int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, int nl, const double* a, const double* macros,
-                       double rtol, double* rhs, void* state, void* ws, void* stream);
+                       double rtol, int64_t cap, double* rhs, void* state, void* ws, void* stream);
 /* u <- P u; r = rhs - K u (reference operator); absolute = |r|.
  * Replaces Resident.certify (resident.py:133-136). */
 int pgj_resident_certify(const pgj_grid* g, int L, int group, int l0, int nl, const double* a, double* u,
                          const double* rhs, double* r, void* state, void* ws, void* stream);
-/* active = |r|^2 > target2 & iterations < 4000 & !bad; first direction
+/* active = |r|^2 > target2 & iterations < cap & !bad; first direction
  * p = M^-1 r (written to p). Replaces Resident.initialize
  * (resident.py:119-123). */
 int pgj_resident_init(const pgj_grid* g, int L, int group, int l0, int nl, const void* plan, const double* s,

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
       for (int c = 0; c < 3; ++c) mean[c] = wave_sum(vm[c]) / double(A.N);
     bool act;
     if (MODE == kFzInit) {
-      act = res2 > A.st.target2[l] && A.st.it[l] < kCap && !A.st.bad[l];
+      act = res2 > A.st.target2[l] && A.st.it[l] < A.st.cap[l] && !A.st.bad[l];
       if (lead) {
         A.st.active_next[l] = act;
         A.st.it_next[l] = A.st.it[l];
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void fz_kernel(FFTArgs A) {
       const bool valid = A.st.valid[l] != 0;  // set by the forward pass of this iteration
       const bool finite = isfinite(res2);
       const int64_t it = A.st.it[l] + (active ? 1 : 0);
-      act = active && valid && finite && res2 > A.st.target2[l] && it < kCap;
+      act = active && valid && finite && res2 > A.st.target2[l] && it < A.st.cap[l];
       if (lead) {
         A.st.active_next[l] = act;
         A.st.it_next[l] = it;

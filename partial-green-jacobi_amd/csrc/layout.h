@@ -14,7 +14,7 @@
 
 namespace pgj {
 
-constexpr int kCap = 4000;   // PCG iteration cap (exp/asg_gpu_resident_20261103/resident.py:50)
+constexpr int kCap = 4000;   // default PCG iteration cap (exp/asg_gpu_resident_20261103/resident.py:50)
 
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 inline size_t align8(size_t v) { return (v + 7) & ~size_t(7); }
@@ -290,7 +290,7 @@ constexpr int kMaxResidentLoads = 4096;
 // Resident state: public blocks first (documented in pgj.h), then private.
 struct StateLayout {
   size_t rz, target2, norm, absolute, it, active, bad;
-  size_t it_next, active_next, bad_next, first, alpha, stopped, valid;
+  size_t it_next, active_next, bad_next, first, alpha, stopped, valid, cap;
   size_t bytes;
 };
 
@@ -311,6 +311,7 @@ inline StateLayout state_layout(int L) {
   s.alpha = off; off += align8(size_t(L) * 8);
   s.stopped = off; off += align8(size_t(L));
   s.valid = off; off += align8(size_t(L));
+  s.cap = off; off += align8(size_t(L) * 8);
   s.bytes = align256(off);
   return s;
 }
@@ -331,6 +332,7 @@ struct StateView {
   double* alpha;
   uint8_t* stopped;   // active at the start of this iteration and stopped by its test
   uint8_t* valid;     // this iteration's p.q and r.z were finite and positive (forward pass)
+  int64_t* cap;       // iteration cap of the load (set by the set-up)
 };
 
 // View of loads [l0, ...) of a state block laid out for L loads.
@@ -352,6 +354,7 @@ inline StateView state_view(void* base, int L, int l0 = 0) {
   v.alpha = reinterpret_cast<double*>(b + s.alpha) + l0;
   v.stopped = reinterpret_cast<uint8_t*>(b + s.stopped) + l0;
   v.valid = reinterpret_cast<uint8_t*>(b + s.valid) + l0;
+  v.cap = reinterpret_cast<int64_t*>(b + s.cap) + l0;
   return v;
 }
 

@@ -32,6 +32,7 @@ __global__ void inner_scale(StateView outer, StateView inner, double* rscale, do
   const double t = fmax(rtol, fmin(need, 0.5));
   inner.target2[l] = t * t;
   inner.it[l] = 0;
+  inner.cap[l] = outer.cap[l];
   inner.bad[l] = 0;
   inner.active[l] = 0;
   inner.first[l] = 0;

@@ -28,14 +28,18 @@ __global__ __launch_bounds__(256) void square_partials(const double* v, int64_t 
   if (threadIdx.x == 0) part[(int64_t(l) * gridDim.x + blockIdx.x) * 4 + 3] = t;
 }
 
-__global__ void setup_finish(const double* part, int nblk, double rtol, double atol, StateView st) {
+__global__ void setup_finish(const double* part, int nblk, double rtol, double atol, int64_t cap, StateView st) {
   const int l = blockIdx.x;
   const double s = wave_reduce_partials(part + int64_t(l) * nblk * 4 + 3, nblk, 4);
   if (threadIdx.x == 0) {
     const double norm = sqrt(s);
-    const double t = fmax(norm * rtol, atol);
+    // max(rtol |f|, atol) that keeps a NaN norm, as torch.maximum (resident.py:191): the load then never
+    // activates and fails certification
+    const double rt = norm * rtol;
+    const double t = (isnan(rt) || rt > atol) ? rt : atol;
     st.norm[l] = norm;
     st.target2[l] = t * t;
+    st.cap[l] = cap;
     st.it[l] = 0;
     st.bad[l] = 0;
     st.active[l] = 0;
@@ -123,12 +127,13 @@ int make_slice(const pgj_grid* g, int L, int group, int l0, int nl, Slice* out) 
 }  // namespace
 
 extern "C" int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, int nl, const double* a,
-                                  const double* macros, double rtol, double* rhs, void* state, void* ws,
-                                  void* stream) {
+                                  const double* macros, double rtol, int64_t cap, double* rhs, void* state,
+                                  void* ws, void* stream) {
   if (int e = check_grid(g)) return e;
   Slice sl{};
   if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
   PGJ_REQUIRE(a && macros && rhs && state && ws, "invalid resident setup arguments");
+  PGJ_REQUIRE(cap >= 1, "iteration cap must be at least 1, got %lld", (long long)cap);
   const int64_t N = sl.N;
   hipStream_t s = as_stream(stream);
   double* f = rhs + sl.vec();
@@ -140,7 +145,7 @@ extern "C" int pgj_resident_setup(const pgj_grid* g, int L, int group, int l0, i
   hipLaunchKernelGGL(square_partials, dim3(nblk, nl), dim3(256), 0, s, f, N, b.pcert);
   PGJ_LAUNCHED("square_partials");
   hipLaunchKernelGGL(setup_finish, dim3(nl), dim3(64), 0, s, b.pcert, nblk, rtol, 1e-13 / sqrt(double(g->nx)),
-                     state_view(state, L, l0));
+                     cap, state_view(state, L, l0));
   PGJ_LAUNCHED("setup_finish");
   return PGJ_OK;
 }

@@ -57,29 +57,52 @@ def split_loads(L, group, parts):
     return None
 
 
+class Fork:
+    """Runs a list of launches, the first on the current stream and each other one on its own side stream forked
+    from and joined back to it (also inside a captured graph). The fork and join events belong to this object and
+    live as long as it does, so an event recorded inside a capture outlives the capture (and the graphs made
+    from it). A fork inside another fork's launches while a capture is open is refused: two serving variants
+    that nested them (and per-class graph replays on concurrent streams) ended the process with a host
+    segmentation fault whose cause was not found. Side streams have the default priority."""
+
+    _depth = 0   # forks currently running their launches (one host thread drives them)
+
+    def __init__(self, sides):
+        self.side = [torch.cuda.Stream() for _ in range(sides)]
+        self.fork = torch.cuda.Event()
+        self.join = [torch.cuda.Event() for _ in range(sides)]
+
+    def run(self, launches):
+        if len(launches) != len(self.side) + 1:
+            raise ValueError(f'{len(launches)} launches for a fork of {len(self.side) + 1} streams')
+        if Fork._depth and torch.cuda.is_current_stream_capturing():
+            raise RuntimeError('a stream fork nested in another one under graph capture is not supported')
+        Fork._depth += 1
+        try:
+            main = torch.cuda.current_stream()
+            self.fork.record(main)
+            for st in self.side:
+                st.wait_event(self.fork)
+            launches[0]()
+            for st, ev, launch in zip(self.side, self.join, launches[1:]):
+                with torch.cuda.stream(st):
+                    launch()
+                ev.record(st)
+                main.wait_event(ev)
+        finally:
+            Fork._depth -= 1
+
+
 class StreamSplit:
-    """Runs ``launch(l0, nl)`` for ``parts`` slices of a set, the first on the current stream and the others on
-    side streams forked from and joined back to it (also inside a captured graph), so the kernels of one slice
-    fill the tails of the others'. Side streams have the default priority."""
+    """Runs ``launch(l0, nl)`` for ``parts`` slices of a set through a ``Fork``, so the kernels of one slice fill
+    the tails of the others'."""
 
     def __init__(self, L, group, parts=2, enable=True):
         self.parts = split_loads(L, group, parts) if enable else None
-        self.side = [torch.cuda.Stream() for _ in range(len(self.parts) - 1)] if self.parts else []
+        self._fork = Fork(len(self.parts) - 1) if self.parts else None
 
     def run(self, launch, L):
         if self.parts is None:
             launch(0, L)
             return
-        main = torch.cuda.current_stream()
-        fork = torch.cuda.Event()
-        fork.record(main)
-        for st in self.side:
-            st.wait_event(fork)
-        l0, nl = self.parts[0]
-        launch(l0, nl)
-        for st, (l0, nl) in zip(self.side, self.parts[1:]):
-            with torch.cuda.stream(st):
-                launch(l0, nl)
-            join = torch.cuda.Event()
-            join.record(st)
-            main.wait_event(join)
+        self._fork.run([lambda l0=l0, nl=nl: launch(l0, nl) for l0, nl in self.parts])

@@ -58,8 +58,8 @@ SIGNATURES = {
     'pgj_resident_state_bytes': (c_size_t, [c_int]),
     'pgj_resident_state_offsets': (c_int, [c_int, ctypes.POINTER(c_size_t)]),
     'pgj_resident_ws_bytes': (c_size_t, [GP, c_int]),
-    'pgj_resident_setup': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_dp, c_double, c_void_p, c_void_p,
-                                   c_void_p, c_void_p]),
+    'pgj_resident_setup': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_dp, c_double, c_int64, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
     'pgj_resident_certify': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p]),
     'pgj_resident_init': (c_int, [GP, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -18,7 +18,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._chunks import MaskPipeline
+from ._chunks import Fork, MaskPipeline
+from .certify import CAP, MAX_RESTARTS, fail, infos as make_infos, outcome
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -27,11 +28,13 @@ LOADS6 = np.eye(6)
 class DesignGroup:
     """D designs of one periodic n^3 grid, each solved for the same G macro loads."""
 
-    def __init__(self, n, designs, loads=LOADS6, chunk=4, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda'):
+    def __init__(self, n, designs, loads=LOADS6, chunk=4, lam=LAMBDA_STUDY2, mu=MU_STUDY2, device='cuda', cap=CAP):
         if chunk < 2 or chunk % 2:
             raise ValueError('graph chunk must be even and >= 2')
+        if int(cap) < 1:
+            raise ValueError('iteration cap must be at least 1')
         self.loads = np.atleast_2d(np.asarray(loads, dtype=np.float64))
-        self.n, self.D, self.G, self.chunk = int(n), int(designs), len(self.loads), chunk
+        self.n, self.D, self.G, self.chunk, self.cap = int(n), int(designs), len(self.loads), chunk, int(cap)
         self.L = self.D * self.G
         shape = (self.n,) * 3
         self.shape = shape
@@ -87,7 +90,8 @@ class DesignGroup:
         l0, nl = self._slice(d0, nd)
         m = np.tile(self.loads, (nl // self.G, 1)).reshape(-1)
         _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.a), _lib.host_doubles(m),
-                  float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
+                  float(rtol), int(self.cap), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
 
     def certify(self, d0=0, nd=None):
         l0, nl = self._slice(d0, nd)
@@ -144,12 +148,14 @@ class BatchedDesigns:
     ``sizes`` maps n to the number of designs of that size.
     """
 
-    def __init__(self, sizes, loads=LOADS6, chunk=4, device='cuda'):
-        self.groups = {int(n): DesignGroup(n, d, loads, chunk, device=device) for n, d in sorted(sizes.items())}
+    def __init__(self, sizes, loads=LOADS6, chunk=4, device='cuda', cap=CAP):
+        self.groups = {int(n): DesignGroup(n, d, loads, chunk, device=device, cap=cap)
+                       for n, d in sorted(sizes.items())}
+        self.cap = int(cap)
         self.chunk = chunk
         self.init_graph = self.iter_graph = self.cert_graph = None
         self._pipeline = None
-        self._side = []   # side streams (default priority) of the concurrent iteration
+        self._forks = {}   # stream forks (default priority) of the concurrent launches, by method name
 
     def _each(self, name):
         def run():
@@ -161,22 +167,12 @@ class BatchedDesigns:
         """The sets' launches on one stream each, forked from and joined to the current stream (the sets share
         no buffers); captured into the iteration graph, so small sets fill the CUs a large set leaves idle."""
         groups = list(self.groups.values())
-        while len(self._side) < len(groups) - 1:
-            self._side.append(torch.cuda.Stream())
-
-        def run():
-            main = torch.cuda.current_stream()
-            fork = torch.cuda.Event()
-            fork.record(main)
-            getattr(groups[0], name)()
-            for g, st in zip(groups[1:], self._side):
-                st.wait_event(fork)
-                with torch.cuda.stream(st):
-                    getattr(g, name)()
-                join = torch.cuda.Event()
-                join.record(st)
-                main.wait_event(join)
-        return run
+        if len(groups) == 1:
+            return getattr(groups[0], name)
+        fork = self._forks.get(name)
+        if fork is None:
+            fork = self._forks[name] = Fork(len(groups) - 1)
+        return lambda: fork.run([getattr(g, name) for g in groups])
 
     def tune(self, reps=6, candidates=None):
         """Time row-march shapes per group on its own buffers and keep the fastest; returns the choices."""
@@ -261,16 +257,14 @@ class BatchedDesigns:
             self._replay(self.cert_graph, 'certify')
             diag = {n: torch.stack((g.norm, g.absolute, g.target2.sqrt(), g.iterations.to(torch.float64),
                                     g.bad.to(torch.float64)), dim=1).cpu().numpy() for n, g in self.groups.items()}
-            ok = {n: np.isfinite(v[:, 1]) & (v[:, 1] <= v[:, 2]) & (v[:, 4] == 0) for n, v in diag.items()}
+            res = {n: outcome(v, self.cap) for n, v in diag.items()}
+            ok = {n: r[0] for n, r in res.items()}
             all_ok = all(o.all() for o in ok.values())
-            hopeless = any(np.any(v[:, 4]) or np.any((~ok[n]) & (v[:, 3] >= 4000)) for n, v in diag.items())
-            if all_ok or restarts >= 2 or hopeless:
+            hopeless = any(r[1] for r in res.values())
+            if all_ok or restarts >= MAX_RESTARTS or hopeless:
                 break
             restarts += 1
-        infos = {n: [dict(rhs_norm=float(v[0]), absolute_residual=float(v[1]), threshold=float(v[2]),
-                          iterations=int(v[3]), relative_residual=float(v[1] / max(v[0], 1e-300)),
-                          converged=bool(ok[n][j]), restarts=restarts) for j, v in enumerate(rows)]
-                 for n, rows in diag.items()}
+        infos = {n: make_infos(rows, ok[n], restarts) for n, rows in diag.items()}
         if not all_ok:
-            raise RuntimeError(('true residual certification failed', infos))
+            fail(infos)
         return infos, host_checks

@@ -16,6 +16,7 @@ import torch
 
 from . import _lib
 from ._chunks import MaskPipeline, StreamSplit
+from .certify import CAP, CertificationError
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
 
 LOADS6 = np.eye(6)
@@ -25,7 +26,7 @@ class MixedResident:
     """D designs x G loads of one cubic n^3 grid solved by FP32-inner / FP64-outer refinement."""
 
     def __init__(self, n, loads=LOADS6, designs=1, inner_rtol=1e-4, chunk=4, max_outer=30, lam=LAMBDA_STUDY2,
-                 mu=MU_STUDY2, device='cuda'):
+                 mu=MU_STUDY2, device='cuda', cap=CAP):
         if chunk < 2 or chunk % 2:
             raise ValueError('graph chunk must be even and >= 2')
         if not 0 < inner_rtol < 1:
@@ -33,7 +34,9 @@ class MixedResident:
         self.loads = np.atleast_2d(np.asarray(loads, dtype=np.float64))
         self.n, self.D, self.G = int(n), int(designs), len(self.loads)
         self.L = self.D * self.G
-        self.inner_rtol, self.chunk, self.max_outer = float(inner_rtol), chunk, int(max_outer)
+        if int(cap) < 1:
+            raise ValueError('iteration cap must be at least 1')
+        self.inner_rtol, self.chunk, self.max_outer, self.cap = float(inner_rtol), chunk, int(max_outer), int(cap)
         shape = (self.n,) * 3
         self.op = Operator(torch.ones(shape, dtype=torch.float64, device=device), None, lam, mu, device).enable_green()
         g = self.op.grid
@@ -103,7 +106,8 @@ class MixedResident:
         l0, nl = self._slice(d0, nd)
         m = np.tile(self.loads, (nl // self.G, 1)).reshape(-1)
         _lib.call('pgj_resident_setup', self.op.grid, self.L, self.G, l0, nl, _lib.ptr(self.a), _lib.host_doubles(m),
-                  float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws), _lib.stream())
+                  float(rtol), int(self.cap), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
 
     def certify(self, d0=0, nd=None):
         l0, nl = self._slice(d0, nd)
@@ -183,20 +187,26 @@ class MixedResident:
             if ok.all():
                 break
             if outer >= self.max_outer or not np.all(np.isfinite(v[:, 1])):
-                raise RuntimeError(('mixed-precision refinement did not certify', outer, v.tolist()))
+                raise CertificationError(('mixed-precision refinement did not certify', self._infos(v, ok, outer)))
             outer += 1
             self._run('begin')
             if self._pipeline is None:
                 self._pipeline = MaskPipeline([self.active32])
             host_checks += self._pipeline.run(lambda: self._run('iteration'))
             if bool(self.bad32.any().item()):
-                raise RuntimeError('FP32 inner PCG broke down (non-positive or non-finite curvature)')
+                raise CertificationError(('FP32 inner PCG broke down (non-positive or non-finite curvature)',
+                                          self._infos(v, ok, outer)))
             inner_total += self.iterations32.cpu().numpy()
             self._run('end_and_certify')
-        infos = [dict(rhs_norm=float(row[0]), absolute_residual=float(row[1]), threshold=float(row[2]),
-                      iterations=int(inner_total[j]), relative_residual=float(row[1] / max(row[0], 1e-300)),
-                      converged=True, outer=outer) for j, row in enumerate(v)]
-        return infos, dict(outer=outer, inner=int(inner_total.sum()), host_checks=host_checks)
+        return self._infos(v, ok, outer, inner_total), dict(outer=outer, inner=int(inner_total.sum()),
+                                                           host_checks=host_checks)
+
+    @staticmethod
+    def _infos(v, ok, outer, inner=None):
+        inner = np.zeros(len(v), dtype=np.int64) if inner is None else inner
+        return [dict(rhs_norm=float(row[0]), absolute_residual=float(row[1]), threshold=float(row[2]),
+                     iterations=int(inner[j]), relative_residual=float(row[1] / max(row[0], 1e-300)),
+                     converged=bool(ok[j]), outer=outer) for j, row in enumerate(v)]
 
     # -- results ------------------------------------------------------------------------
     def solutions(self, d):

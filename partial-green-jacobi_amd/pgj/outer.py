@@ -195,6 +195,11 @@ class GaussianOuter(_Fields):
                       _lib.stream())
         self._change()
 
+    def capture(self):
+        """The reference captures this loop into graphs (``outer.py:46-54``); here beta is a launch argument of the
+        projection, so the loop runs eagerly on the current stream (the same kernels, the same results)."""
+        return self
+
     def evaluate(self, beta):
         self.beta = float(beta)
         self.physical(beta)

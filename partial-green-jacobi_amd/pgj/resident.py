@@ -4,9 +4,10 @@ Mirrors ``exp/asg_gpu_resident_20261103/resident.py:61-213`` (``Resident``)
 and the 6-load grouping/energy/C^H of
 ``exp/asg_sctop_reproduction_20261105/reproduction.py:121-176``
 (``PaperResident``): per-load independent recurrences with device-latched
-masks, an iteration cap of 4000, up to two restarts from the certified
-residual, true-residual certification ``|f - K u| <= max(rtol |f|, 1e-13/sqrt(n))``
-and ``RuntimeError`` on failure. Only per-load scalars cross to the host, once
+masks, an iteration cap (4000 by default, a per-load device field), up to two
+restarts from the certified residual, true-residual certification
+``|f - K u| <= max(rtol |f|, 1e-13/sqrt(n))`` and ``CertificationError`` (a
+``RuntimeError``, as the reference raises) on failure. Only per-load scalars cross to the host, once
 per ``chunk`` iterations.
 """
 import math
@@ -16,21 +17,52 @@ import torch
 
 from . import _lib
 from ._chunks import MaskPipeline, StreamSplit
+from .certify import CAP, MAX_RESTARTS, fail, infos as make_infos, outcome
+from .backend import lame
 from .operator import LAMBDA_STUDY2, MU_STUDY2, Operator
+from .outer import GaussianOuter, PaperOuter
 
 LOADS6 = np.eye(6)
 
 
 class Resident:
-    """Solves L unit-strain loads on one periodic grid, ``batch`` loads per recurrence group.
+    """Solves L macro-strain loads on one periodic grid, ``batch`` loads per recurrence group.
+
+    Two ways to build it:
+
+    - the reference's, ``Resident(backend, n, chunk=4, load_mode='serial')``
+      (``exp/asg_gpu_resident_20261103/resident.py:61-96``): ``backend`` is any
+      object with an isotropic 6x6 ``C0`` and ``loads`` (L, 6) (``pgj.backend``);
+      the Study-I outer loop (``GaussianOuter``, ``outer.py:12-59``) is attached
+      as ``self.outer`` and captured with the solver;
+    - ``Resident(n, loads=..., load_mode=..., batch=..., lam=..., mu=...)``:
+      the grid size first, everything else by keyword, no outer loop.
 
     ``load_mode='serial'`` solves one load at a time (batch 1), ``'parallel'``
     groups loads (batch 3 by default, as the reference; any divisor of L may
-    be given with ``batch``).
+    be given with ``batch``). ``cap`` is the per-load iteration cap
+    (the reference's 4000).
     """
 
-    def __init__(self, n, loads=LOADS6, load_mode='serial', batch=None, chunk=4, lam=LAMBDA_STUDY2,
-                 mu=MU_STUDY2, device='cuda', streams=2):
+    def __init__(self, backend, n=None, chunk=4, load_mode='serial', *, loads=None, batch=None, lam=None, mu=None,
+                 device='cuda', streams=2, cap=CAP, outer=True):
+        if hasattr(backend, 'C0') and hasattr(backend, 'loads'):
+            if n is None:
+                raise TypeError('Resident(backend, n, chunk, load_mode) needs the grid size n')
+            blam, bmu = lame(backend.C0)
+            lam = blam if lam is None else lam
+            mu = bmu if mu is None else mu
+            loads = backend.loads if loads is None else loads
+            self.b = backend
+        else:
+            if n is not None:
+                raise TypeError('Resident(n, ...) takes the grid size first and the rest by keyword '
+                                '(loads=, load_mode=, batch=), or the reference form Resident(backend, n, chunk, '
+                                'load_mode)')
+            n, self.b, outer = backend, None, False
+            lam = LAMBDA_STUDY2 if lam is None else lam
+            mu = MU_STUDY2 if mu is None else mu
+            loads = LOADS6 if loads is None else loads
         if load_mode not in ('serial', 'parallel'):
             raise ValueError('load_mode must be serial or parallel')
         if chunk < 2 or chunk % 2:
@@ -40,7 +72,9 @@ class Resident:
         self.batch = 1 if load_mode == 'serial' else (batch or min(3, nl))
         if nl % self.batch:
             raise ValueError('batch must divide the number of loads')
-        self.load_mode, self.chunk = load_mode, chunk
+        if int(cap) < 1:
+            raise ValueError('iteration cap must be at least 1')
+        self.load_mode, self.chunk, self.cap = load_mode, chunk, int(cap)
         self.n = int(n) if np.isscalar(n) else tuple(int(v) for v in n)
         shape = (n, n, n) if np.isscalar(n) else tuple(n)
         self.shape = shape
@@ -78,6 +112,10 @@ class Resident:
         self.ch_ws = torch.empty(lib.pgj_effective_tensor_ws_bytes(g) // 8 + 1, dtype=torch.float64, device=dev)
         self.ch_out = z(36)
         self.init_graph = self.iter_graph = self.cert_graph = None
+        self.outer = None
+        if outer and isinstance(self.n, int):
+            # Study-I outer loop of the reference's Resident (Gaussian filter, tanh projection, sqrt OC)
+            self.outer = GaussianOuter(self.n, device=device)
 
     # -- kernels ---------------------------------------------------------------------
     def to_soa(self, v):
@@ -112,7 +150,7 @@ class Resident:
         m = self.loads[list(load_indices)].reshape(-1)
         nl = len(load_indices)
         _lib.call('pgj_resident_setup', self.op.grid, nl, nl, 0, nl, _lib.ptr(self.op._material_field),
-                  _lib.host_doubles(m), float(rtol), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.host_doubles(m), float(rtol), int(self.cap), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
                   _lib.stream())
 
     def certify(self):
@@ -146,7 +184,9 @@ class Resident:
         self._halves.run(lambda l0, nl: self._iterate_slice(l0, nl, count), self.batch)
 
     def capture(self):
-        """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``)."""
+        """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``), and the outer loop's."""
+        if self.outer is not None:
+            self.outer.capture()
         self._setup(range(self.batch), 1e-8)
         self.r.normal_()
         self.certify()
@@ -222,15 +262,13 @@ class Resident:
             self.cert_graph.replay()
             diag = torch.stack((self.norm, self.absolute, self.target2.sqrt(), self.iterations.to(torch.float64),
                                 self.bad.to(torch.float64)), dim=1).cpu().numpy()
-            ok = np.isfinite(diag[:, 1]) & (diag[:, 1] <= diag[:, 2]) & (diag[:, 4] == 0)
-            if ok.all() or restarts >= 2 or np.any(diag[:, 4]) or np.any((~ok) & (diag[:, 3] >= 4000)):
+            ok, stop = outcome(diag, self.cap)
+            if ok.all() or restarts >= MAX_RESTARTS or stop:
                 break
             restarts += 1
-        info = [dict(rhs_norm=float(v[0]), absolute_residual=float(v[1]), threshold=float(v[2]),
-                     iterations=int(v[3]), relative_residual=float(v[1] / max(v[0], 1e-300)), converged=bool(ok[j]),
-                     restarts=restarts) for j, v in enumerate(diag)]
+        info = make_infos(diag, ok, restarts)
         if not ok.all():
-            raise RuntimeError(('true residual certification failed', info))
+            fail(info)
         return info, host_checks
 
     # -- responses ---------------------------------------------------------------------
@@ -251,8 +289,16 @@ class Resident:
                   _lib.ptr(self.ch_ws), _lib.ptr(self.ch_out), _lib.stream())
         return self.ch_out.reshape(6, 6)
 
-    def energy(self, objective='shear', iteration=0):
-        """(objective, local sensitivity density, diagonal moduli) (``reproduction.py:153-164``)."""
+    def energy(self):
+        """(moduli (L,), sensitivity (n, n, n)): the a-weighted mean energy per load and the load-mean element
+        energy (Study I, ``exp/asg_gpu_resident_20261103/resident.py:211-213``)."""
+        e = self.energies()
+        a = self.op._material_field.reshape(1, -1)
+        return (e * a).mean(1), e.mean(0).reshape(self.shape)
+
+    def objective_energy(self, objective='shear', iteration=0):
+        """(objective, local sensitivity density, diagonal moduli) of a Study-II objective
+        (``reproduction.py:153-164``)."""
         e = self.energies()
         a = self.op._material_field.reshape(1, -1)
         diagonal = (e * a).mean(1)
@@ -269,3 +315,29 @@ class Resident:
                 raise ValueError(objective)
         obj = (local * a.reshape(-1)).mean()
         return obj, local.reshape(self.shape), diagonal
+
+
+class PaperResident(Resident):
+    """The Study-II resident solver (``exp/asg_sctop_reproduction_20261105/reproduction.py:121-176``).
+
+    ``PaperResident(backend, n, objective, volume, rmin=3., beta=4., eta=.5, emin=1e-9, load_mode='parallel',
+    chunk=4)``: all of ``backend.loads`` (the six unit strains) in groups of 3 (parallel) or 1 (serial), the
+    cone-filter / eta-projection OC loop (``PaperOuter``) as ``self.outer`` with the reference's move limit 0.1
+    unless ``move`` is given, ``energy(iteration) -> (objective, local, diagonal)`` for 'shear', 'bulk' or
+    'auxetic', and ``effective_tensor()``.
+    """
+
+    def __init__(self, backend, n, objective, volume, rmin=3., beta=4., eta=.5, emin=1e-9, load_mode='parallel',
+                 chunk=4, *, move=0.1, batch=None, device='cuda', streams=2, cap=CAP):
+        if load_mode not in ('serial', 'parallel'):
+            raise ValueError(load_mode)
+        if objective not in ('shear', 'bulk', 'auxetic'):
+            raise ValueError(objective)
+        super().__init__(backend, n, chunk, load_mode, batch=batch, device=device, streams=streams, cap=cap,
+                         outer=False)
+        self.objective_kind = objective
+        self.outer = PaperOuter(int(n), volume, rmin=rmin, beta=beta, eta=eta, emin=emin, move=move, device=device)
+
+    def energy(self, iteration=0):
+        """(objective, local (n, n, n), diagonal (6,)) (``reproduction.py:153-164``)."""
+        return self.objective_energy(self.objective_kind, iteration)

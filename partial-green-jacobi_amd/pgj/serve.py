@@ -32,6 +32,7 @@ import torch
 
 from . import _lib
 from .batched import DesignGroup
+from .certify import CAP, MAX_RESTARTS, CertificationError, outcome
 from .density import filtered_noise
 from .mixed import MixedResident
 from .outer import PaperOuter
@@ -66,6 +67,7 @@ class Client:
             int(seed), int(steps)
         self.x = torch.tensor(filtered_noise(self.n, self.seed, radius), device=device)
         self.done = 0
+        self.failed = 0     # requests whose solve failed certification (the design stays as it was)
         self.slot = None
         self.busy = False
         self.history = []   # per completed step: dict(CH, iterations, restarts, change, latency)
@@ -76,19 +78,21 @@ class Request:
         self.client, self.arrival = client, arrival
         self.start = self.end = None
         self.restarts = 0
+        self.failed = False
 
 
 class Server:
     """Design-step server: ``sizes`` are the grid sizes served, ``budget`` the HBM bytes for resident sets."""
 
     def __init__(self, sizes, budget, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, chunk=4, max_slots=16,
-                 device='cuda', graphs=True, precision='fp64', inner_rtol=1e-4, max_outer=30):
+                 device='cuda', graphs=True, precision='fp64', inner_rtol=1e-4, max_outer=30, cap=CAP):
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.sizes = sorted(int(n) for n in sizes)
         self.rtol, self.chunk, self.device, self.precision = rtol, chunk, device, precision
         self.volume, self.move, self.rmin = volume, move, rmin
-        self.max_outer = max_outer
+        self.max_outer, self.cap = max_outer, int(cap)
+        self.failed = []   # (client id, grid, Emin, alpha) of requests whose solve failed certification
         share = budget / len(self.sizes)
         self.groups = {}
         for n in self.sizes:
@@ -96,9 +100,10 @@ class Server:
             if slots < 1:
                 raise ValueError(f'HBM budget {budget / 2**30:.1f} GiB holds no slot of {n}^3')
             if precision == 'mixed':
-                self.groups[n] = MixedResident(n, LOADS6, slots, inner_rtol, chunk, max_outer, device=device)
+                self.groups[n] = MixedResident(n, LOADS6, slots, inner_rtol, chunk, max_outer, device=device,
+                                               cap=cap)
             else:
-                self.groups[n] = DesignGroup(n, slots, LOADS6, chunk, device=device)
+                self.groups[n] = DesignGroup(n, slots, LOADS6, chunk, device=device, cap=cap)
         self.free = {n: list(range(g.D))[::-1] for n, g in self.groups.items()}
         self.owner = {n: [None] * g.D for n, g in self.groups.items()}
         self.solving = {n: {} for n in self.sizes}          # slot -> Request
@@ -193,13 +198,15 @@ class Server:
         return out
 
     def _complete_mixed(self, n, slot):
-        """One refinement boundary of a mixed-precision design: returns the (6, 5) status rows once certified."""
+        """One refinement boundary of a mixed-precision design: the (6, 5) status rows once certified, None
+        while it refines, False when it cannot certify (a broken-down inner solve, a non-finite residual, or
+        ``max_outer`` steps spent)."""
         g = self.groups[n]
         req = self.solving[n][slot]
         if not req.outer_pending:
             sl = slice(6 * slot, 6 * slot + 6)
             if bool(g.bad32[sl].any().item()):
-                raise RuntimeError(('FP32 inner PCG broke down', req.client.cid))
+                return False
             req.inner += g.iterations32[sl].cpu().numpy()
             g.end(slot, 1)
             g.certify(slot, 1)
@@ -209,33 +216,54 @@ class Server:
         if ok.all():
             return np.column_stack((v, req.inner.astype(np.float64), np.zeros(6)))
         if req.outer >= self.max_outer or not np.all(np.isfinite(v[:, 1])):
-            raise RuntimeError(('mixed-precision refinement did not certify', req.client.cid, v.tolist()))
+            return False
         req.outer += 1
         g.begin(slot, 1)
         return None
 
     def _complete(self, n, slot, t_end):
+        """Advance a design whose loads all stopped: True once its step is answered (certified or failed)."""
         g = self.groups[n]
         req = self.solving[n][slot]
-        c = req.client
         if self.precision == 'mixed':
             diag = self._complete_mixed(n, slot)
             if diag is None:
                 return False
+            if diag is False:
+                return self._fail(n, slot, t_end)
             return self._finalize(n, slot, t_end, diag)
         g.certify(slot, 1)
         sl = slice(6 * slot, 6 * slot + 6)
         diag = torch.stack((g.norm[sl], g.absolute[sl], g.target2[sl].sqrt(), g.iterations[sl].to(torch.float64),
                             g.bad[sl].to(torch.float64)), dim=1).cpu().numpy()
-        ok = np.isfinite(diag[:, 1]) & (diag[:, 1] <= diag[:, 2]) & (diag[:, 4] == 0)
+        ok, stop = outcome(diag, self.cap)
         if not ok.all():
-            hopeless = np.any(diag[:, 4]) or np.any((~ok) & (diag[:, 3] >= 4000))
-            if req.restarts >= 2 or hopeless:
-                raise RuntimeError(('true residual certification failed', c.cid, diag.tolist()))
+            if req.restarts >= MAX_RESTARTS or stop:
+                return self._fail(n, slot, t_end)
             req.restarts += 1
             g.initialize(slot, 1)
             return False
         return self._finalize(n, slot, t_end, diag)
+
+    def _fail(self, n, slot, t_end):
+        """A step that failed certification: record it, keep the client's design and warm start, free the slot
+        if the client has no step left, and go on (as ``SequentialServer`` does)."""
+        req = self.solving[n][slot]
+        c = req.client
+        req.end = t_end
+        req.failed = True
+        self.failed.append((c.cid, c.n, c.emin, c.alpha))
+        c.failed += 1
+        c.busy = False
+        del self.solving[n][slot]
+        self._release(n, slot, c)
+        return True
+
+    def _release(self, n, slot, c):
+        if c.done + c.failed >= c.steps:
+            self.owner[n][slot] = None
+            self.free[n].append(slot)
+            c.slot = None
 
     def _finalize(self, n, slot, t_end, diag):
         """C^H, energies, OC update and the response of a certified design step."""
@@ -261,10 +289,7 @@ class Server:
         c.done += 1
         c.busy = False
         del self.solving[n][slot]
-        if c.done >= c.steps:
-            self.owner[n][slot] = None
-            self.free[n].append(slot)
-            c.slot = None
+        self._release(n, slot, c)
         return True
 
     # -- the loop -----------------------------------------------------------------------
@@ -303,14 +328,14 @@ class Server:
             t_start = time.perf_counter()
             for req in pending:
                 c = req.client
-                if c.busy or c.done >= c.steps or not self._admit(c):
+                if c.busy or c.done + c.failed >= c.steps or not self._admit(c):
                     continue
                 c.busy = True
                 req.start = now()
                 self._start(req)
                 started.append(req)
             self.phases['start'] += time.perf_counter() - t_start
-            pending = [r for r in pending if r not in started and r.client.done < r.client.steps]
+            pending = [r for r in pending if r not in started and r.client.done + r.client.failed < r.client.steps]
             if not any(self.solving[n] for n in self.sizes):
                 if heap:
                     wait = heap[0][0] - now()
@@ -328,7 +353,7 @@ class Server:
             t3 = time.perf_counter()
             for n, slot in finished:
                 req = self.solving[n][slot]
-                if self._complete(n, slot, now()):
+                if self._complete(n, slot, now()) and not req.failed:
                     done.append(req)
             t4 = time.perf_counter()
             self.phases['advance'] += t2 - t1
@@ -348,16 +373,17 @@ class SequentialServer:
     starts when the previous one has finished.
     """
 
-    def __init__(self, sizes, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, device='cuda'):
+    def __init__(self, sizes, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, device='cuda', cap=CAP):
         from .resident import Resident
         self.rtol, self.device = rtol, device
         self.volume, self.move, self.rmin = volume, move, rmin
-        self.solvers = {int(n): Resident(int(n), LOADS6, 'parallel', batch=6, device=device).capture()
+        self.solvers = {int(n): Resident(int(n), loads=LOADS6, load_mode='parallel', batch=6, device=device, cap=cap).capture()
                         for n in sizes}
         self.workers = {}
         self.warm = {}
         self.unserved = 0
         self.failed = []   # (client id, grid, Emin, alpha) of requests whose solve failed certification
+        self.failed_seconds = 0.0   # wall time spent on those requests
 
     def _worker(self, n, emin):
         w = self.workers.get((n, emin))
@@ -377,7 +403,7 @@ class SequentialServer:
         res.material(w.a)
         res.select(c.alpha)
         infos, _ = res.solve(rtol=self.rtol)
-        _, local, _ = res.energy('shear')
+        _, local, _ = res.objective_energy('shear')
         H = res.effective_tensor()
         count = w.step(local)
         c.x.copy_(w.new)
@@ -403,12 +429,14 @@ class SequentialServer:
             req.start = now()
             try:
                 infos, H, count = self._step(clients[cid])
-            except RuntimeError:
-                # the plain path hit the 4000-iteration cap short of the tolerance (the reference raises
-                # there too): the request fails, the client keeps its design, the queue goes on
+            except CertificationError:
+                # the solve did not certify (the reference raises there too): the request fails, the client
+                # keeps its design, the queue goes on
                 torch.cuda.synchronize()
                 c = clients[cid]
+                c.failed += 1
                 self.failed.append((cid, c.n, c.emin, c.alpha))
+                self.failed_seconds += now() - req.start
                 continue
             torch.cuda.synchronize()
             req.end = now()

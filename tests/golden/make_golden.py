@@ -1,6 +1,8 @@
 """Writes the small golden fixtures under tests/golden/ from the oracle.
 
-Run from the repository root: ``python tests/golden/make_golden.py``. Every
+Run from the repository root: ``python tests/golden/make_golden.py`` (the
+small fixtures), or name fixtures, e.g. ``python tests/golden/make_golden.py
+solve_64_alpha`` (the 64^3 solves take minutes of CPU). Every
 input is seeded; outputs are FP64 arrays in compressed .npz files (no pickled
 objects). The reference repository holds no numeric fixtures for this path, so
 these record the oracle's outputs (the oracle itself is pinned by closed forms
@@ -80,12 +82,35 @@ def oc_fixture():
     return dict(x=x, energy=energy, new=new, count=np.array(count), g=g, dv=dv)
 
 
+def solve64_fixture():
+    """Plain method at 64^3 (Emin 1e-6, box radius 4, seed 20261104, rtol 1e-8, zero start) at alpha 0, 0.5
+    and 1: per-load iterations, relative residuals and C^H. The recurrences are per load, so these are the
+    serial and the parallel load mode's numbers alike (tests/test_oracle_pcg.py checks serial = parallel)."""
+    n = 64
+    a = design.simp(design.filtered_noise(n, 20261104, 4), 1e-6)
+    out = {}
+    for tag, alpha in (('a0', 0.0), ('a05', 0.5), ('a1', 1.0)):
+        s = green.scale(a, alpha, C0)
+        ref = pcg.ResidentPCG(a, C0, np.eye(6), batch=6, s=s)
+        t = time.time()
+        infos, _ = ref.solve(rtol=1e-8)
+        its = [i['iterations'] for i in infos]
+        print(f'64^3 alpha {alpha}: {time.time() - t:.1f} s, iterations {its}', flush=True)
+        out['iterations_' + tag] = np.array(its)
+        out['relative_residual_' + tag] = np.array([i['relative_residual'] for i in infos])
+        out['CH_' + tag] = design.effective_tensor(a, ref.solutions, C0)
+    return out
+
+
 FIXTURES = {'operator_4x5x6': operator_fixture, 'green_3x4x5': green_fixture, 'solve_16_alpha05': solve_fixture,
             'oc_cone_8': oc_fixture}
+# minutes of CPU: written only when named on the command line
+LARGE = {'solve_64_alpha': solve64_fixture}
 
 
 if __name__ == '__main__':
-    for name, fn in FIXTURES.items():
+    chosen = {k: v for k, v in {**FIXTURES, **LARGE}.items() if k in sys.argv[1:]} or FIXTURES
+    for name, fn in chosen.items():
         path = os.path.join(HERE, name + '.npz')
         np.savez_compressed(path, **fn())
         print(name, os.path.getsize(path), 'bytes')

@@ -134,7 +134,7 @@ def test_resident_large_certifies(n):
     """Compile-time FFT and two-x-per-lane HEX8 at 96^3 / 128^3: one load group certified, rechecked on the CPU."""
     print(f'resident large n={n}', flush=True)
     a = density(n, 1e-4, 4)
-    r = pgj.Resident(n, np.eye(6)[[0, 3]], load_mode='parallel', batch=2).capture()
+    r = pgj.Resident(n, loads=np.eye(6)[[0, 3]], load_mode='parallel', batch=2).capture()
     r.material(torch.tensor(a, device='cuda'))
     r.select('0.5')
     infos, _ = r.solve()

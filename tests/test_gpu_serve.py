@@ -58,7 +58,7 @@ def test_client_matches_plain_path():
     server = Server((16,), BUDGET, max_slots=2)
     server.run(c, [(0.0, 0), (0.001, 0), (0.002, 0)])
     n, emin, alpha, seed = SPECS[0]
-    res = pgj.Resident(n, np.eye(6), 'parallel', batch=6).capture()
+    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=6).capture()
     outer = PaperOuter(n, 0.35, rmin=3.0, beta=4.0, eta=0.5, emin=emin, move=0.2).capture()
     outer.set_initial(Client(0, n, emin, alpha, seed, 1).x)
     for k in range(STEPS):
@@ -66,7 +66,7 @@ def test_client_matches_plain_path():
         res.material(outer.a)
         res.select(alpha)
         infos, _ = res.solve(rtol=1e-8)
-        _, local, _ = res.energy('shear')
+        _, local, _ = res.objective_energy('shear')
         H = res.effective_tensor().cpu().numpy()
         count = outer.step(local)
         outer.advance()
