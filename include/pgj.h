@@ -82,6 +82,20 @@ size_t pgj_effective_tensor_ws_bytes(const pgj_grid* g);
 int pgj_effective_tensor(const pgj_grid* g, const double* a, const double* u6, double* ws, double* out36,
                          void* stream);
 
+/* C^H as pgj_effective_tensor, with no workspace argument and no allocation. No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The per-block partial sums live in a file-scope device array of
+ * 2,752,512 bytes, one per GPU, held until the process ends; its address is
+ * looked up on the first call on each device (make that call outside a
+ * capture). Grids of more than pgj_effective_tensor_noalloc_max_elements()
+ * (4,194,304) elements are rejected with PGJ_ERR_ARG. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams or threads)
+ * race; calls ordered on one stream are safe. */
+int64_t pgj_effective_tensor_noalloc_max_elements(void);
+int pgj_effective_tensor_noalloc(const pgj_grid* g, const double* a, const double* u6, double* out36,
+                                 void* stream);
+
 /* Mean stress <a C0 (eps(macro) + B_bar u)> (6 doubles). Replaces `stress`
  * (solver/src/materials/hybrid_kernels.cu:138-147). */
 size_t pgj_avg_stress_ws_bytes(const pgj_grid* g);

@@ -2,6 +2,7 @@
 // the independent thread-per-node reference apply, rhs, energies, C^H,
 // mean stress, nodal diagonal / partial Green-Jacobi scale and projection.
 #include <cmath>
+#include <mutex>
 
 #include "hex8_kernels.h"
 
@@ -936,6 +937,73 @@ extern "C" int pgj_effective_tensor(const pgj_grid* g, const double* a, const do
                        ws);
   PGJ_LAUNCHED("effective_partials");
   hipLaunchKernelGGL(effective_finish, dim3(21), dim3(64), 0, s, ws, nblk, N, out36);
+  PGJ_LAUNCHED("effective_finish");
+  return PGJ_OK;
+}
+
+// ---- C^H without a caller workspace -----------------------------------------------------------------
+// The partial sums live in a file-scope device array, one copy per GPU. Host code never passes the array's
+// name to a kernel: in host code it is only a stand-in for the symbol. Its device address is looked up once
+// per device (hipGetSymbolAddress, with that device current) and kept in `g_scratch_ptr`.
+static constexpr int64_t kNoAllocMaxElements = int64_t(1) << 22;                 // 4,194,304 elements
+static constexpr size_t kNoAllocScratchDoubles = size_t(kNoAllocMaxElements / 256) * 21;  // 2,752,512 bytes
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("hipGetDevice failed: %s", hipGetErrorString(e));
+    return PGJ_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d beyond the %d devices the scratch table holds", dev, kMaxDevices);
+    return PGJ_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return PGJ_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PGJ_OK;
+}
+}  // namespace
+
+extern "C" int64_t pgj_effective_tensor_noalloc_max_elements(void) { return kNoAllocMaxElements; }
+
+extern "C" int pgj_effective_tensor_noalloc(const pgj_grid* g, const double* a, const double* u6, double* out36,
+                                            void* stream) {
+  if (int e = check_grid(g)) return e;
+  PGJ_REQUIRE(a && u6 && out36, "invalid effective tensor arguments");
+  const int64_t N = int64_t(g->nx) * g->ny * g->nz;
+  PGJ_REQUIRE(N <= kNoAllocMaxElements,
+              "pgj_effective_tensor_noalloc: grid of %lld elements exceeds the %lld-element limit of its "
+              "built-in scratch; use pgj_effective_tensor with a workspace", (long long)N,
+              (long long)kNoAllocMaxElements);
+  double* part = nullptr;
+  if (int e = scratch_for_current_device(&part)) return e;
+  // every block overwrites its own 21 partials before effective_finish reads them: nothing to clear
+  const int nblk = int((N + 255) / 256);
+  hipStream_t s = as_stream(stream);
+  if (cubic(*g))
+    hipLaunchKernelGGL(effective_partials<true>, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a,
+                       u6, part);
+  else
+    hipLaunchKernelGGL(effective_partials<false>, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), a,
+                       u6, part);
+  PGJ_LAUNCHED("effective_partials");
+  hipLaunchKernelGGL(effective_finish, dim3(21), dim3(64), 0, s, part, nblk, N, out36);
   PGJ_LAUNCHED("effective_finish");
   return PGJ_OK;
 }

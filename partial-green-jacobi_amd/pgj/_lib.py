@@ -36,6 +36,8 @@ SIGNATURES = {
     'pgj_hex8_energy': (c_int, [GP, c_int, c_void_p, c_dp, c_void_p, c_void_p]),
     'pgj_effective_tensor_ws_bytes': (c_size_t, [GP]),
     'pgj_effective_tensor': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_effective_tensor_noalloc_max_elements': (c_int64, []),
+    'pgj_effective_tensor_noalloc': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_avg_stress_ws_bytes': (c_size_t, [GP]),
     'pgj_hex8_avg_stress': (c_int, [GP, c_void_p, c_void_p, c_dp, c_void_p, c_void_p, c_void_p]),
     'pgj_scale_ws_bytes': (c_size_t, [GP]),
