@@ -24,10 +24,11 @@ import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of each stage at 64^3 with 6 loads, from the FETCH_SIZE / WRITE_SIZE counter runs of
-# `bench.py --profile-stages` scaled by a 256 MiB copy (read factor 2.0, write factor 1.0);
-# profiles/pcg_stages_64cube_6loads_traffic.md holds the commands and the table.
-MEASURED_TRAFFIC_64_6 = {'hex8_rowmarch': 93.7e6, 'fxy': 162.7e6, 'fz': 121.6e6, 'fxyi': 208.6e6}
+# HBM bytes per launch of each stage in the timed solve's launch shape (64^3, a 3-load slice of the 6-load set),
+# from the FETCH_SIZE / WRITE_SIZE counter runs of `bench.py --profile-stages` scaled by a 256 MiB copy;
+# profiles/pcg_stages_64cube_3load_slices_traffic.md holds the commands, the calibration and the table.
+# Keyed by (n, loads in the set, loads per launch).
+MEASURED_TRAFFIC = {(64, 6, 3): {'hex8_rowmarch': 47.89e6, 'fxy': 82.73e6, 'fz': 61.41e6, 'fxyi': 103.36e6}}
 
 
 def parse():
@@ -47,9 +48,15 @@ def parse():
     p.add_argument('--probe-iters', type=int, default=10)
     p.add_argument('--streams', type=int, default=2,
                    help='streams the parallel load group is split over in the solve (slices of batch/streams loads)')
-    p.add_argument('--cpu-seconds', type=float, default=15.0)
+    p.add_argument('--cpu-seconds', type=float, default=12.0)
     p.add_argument('--no-cpu', action='store_true')
-    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid', 'configs'), default='plain',
+    p.add_argument('--tune', type=int, default=1, help='1: time row-march shapes and FFT radix plans first')
+    p.add_argument('--no-forms', action='store_true',
+                   help='skip the served 64^3 batched mixed-precision stream against the plain path')
+    p.add_argument('--served-designs', type=int, default=16)
+    p.add_argument('--served-steps', type=int, default=3)
+    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid', 'configs', 'served64'),
+                   default='plain',
                    help='plain: the FP64 drop-in design step (default); batched: 48 designs of uneven grids in one '
                         'captured graph vs the same designs solved one at a time; serve: Poisson-arrival '
                         'design-step requests through the continuous-batching server; mixed: FP32-inner / '
@@ -79,19 +86,22 @@ def stage_bytes(L, n):
     }
 
 
-def probe_stages(res, iters):
-    """Average device duration of each stage with all loads active: after one full iteration, each stage is
-    launched ``iters`` times back to back between two HIP events on the launch stream (a repeated stage keeps
-    every load active; the values it leaves are not used), so the launch gaps are amortised."""
+def probe_stages(res, iters, nl=None):
+    """Average device duration of each stage with all loads active, for launches over the load slice [0, nl) of
+    the set (the solve's launch shape: with the two-stream split each launch covers half the group): after one
+    full iteration, each stage is launched ``iters`` times back to back between two HIP events on the launch
+    stream (a repeated stage keeps every load active; the values it leaves are not used), so the launch gaps
+    are amortised."""
     from pgj import _lib
     names = ['hex8_rowmarch', 'fxy', 'fz', 'fxyi']
+    nl = res.batch if nl is None else nl
     res._setup(range(res.batch), 1e-8)
     res.u.zero_()
     res.certify()
     res.initialize()
 
     def launch(stage):
-        _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, 0, res.batch, _lib.ptr(res.op._plan),
+        _lib.call('pgj_resident_stage', res.op.grid, res.batch, res.batch, 0, nl, _lib.ptr(res.op._plan),
                   _lib.ptr(res.op._material_field), _lib.ptr(res.scale), _lib.ptr(res.scale_hat),
                   _lib.ptr(res.u), _lib.ptr(res.r), _lib.ptr(res.p0), _lib.ptr(res.p1), _lib.ptr(res.q),
                   _lib.ptr(res.state), _lib.ptr(res.ws), stage, _lib.stream())
@@ -126,40 +136,57 @@ def host_info():
     return {'cpu_model': model, 'os_cpu_count': os.cpu_count()}
 
 
-def cpu_baseline(a, alpha, its_total, seconds):
-    """Oracle (numpy FP64) resident PCG on a bounded sample of the same solve."""
+def cpu_baseline(a, alpha, load_iterations, seconds, runs=3):
+    """Oracle (numpy FP64) resident PCG on a bounded sample of the same solve, timed ``runs`` times.
+
+    One run: the oracle's six-load group (parallel load mode, as the GPU step) from a zero start, iterated for
+    about ``seconds``; its time per load-iteration, the median over the runs, is scaled to
+    ``load_iterations`` (the timed GPU steps' mean) plus the 12 certification applies of a step. The FFTs run
+    through scipy.fft with one worker per host CPU; the operator's elementwise numpy runs on one thread."""
     from threadpoolctl import threadpool_info
     from oracle import green, hex8, pcg
-    C0 = hex8.isotropic(15 / 26, 5 / 13)
-    s = green.scale(a, alpha, C0)
-    solver = pcg.ResidentPCG(a, C0, np.eye(6)[3:4], batch=1, s=s)
-    f = solver.rhs(np.eye(6)[3])
-    t0 = time.perf_counter()
-    u = hex8.project(np.zeros_like(f))
-    r = f - solver.true_K(u)
-    t_cert = time.perf_counter() - t0
-    norm = np.sqrt((f * f).sum(1))
-    target2 = np.maximum(norm * 1e-8, 1e-13 / math.sqrt(a.shape[0])) ** 2
-    active = (r * r).sum(1) > target2
-    z = solver.B(r)
-    p = np.where(active[:, None], z, 0.0)
-    rz = (r * z).sum(1)
-    it = np.zeros(1, np.int64)
-    bad = np.zeros(1, bool)
-    t0 = time.perf_counter()
-    iters = 0
-    while time.perf_counter() - t0 < seconds and active.any():
-        u, r, p, rz, active, it, bad = solver.iteration(u, r, p, rz, active, it, bad, target2)
-        iters += 1
-    t_iter = (time.perf_counter() - t0) / max(iters, 1)
-    per_solve = t_iter * its_total + 2 * 6 * t_cert
-    threads = max([i.get('num_threads', 1) for i in threadpool_info()] or [1])
-    return {'value': 1.0 / per_solve, 'unit': 'certified 6-load solves/s', 'cores': int(threads), 'kind': 'port',
-            'host': host_info(),
-            'sample': f'{iters} iterations of the oracle resident PCG on load 3 (yz shear) of the first design '
-                      f'step at {a.shape[0]}^3 (numpy FP64, pocketfft single-threaded, BLAS {threads} threads), '
-                      f'{t_iter * 1e3:.1f} ms/iteration, extrapolated to the GPU step\'s {its_total} total '
-                      f'iterations plus 12 certification applies'}
+    workers = os.cpu_count() or 1
+    green.FFT_WORKERS = workers
+    try:
+        C0 = hex8.isotropic(15 / 26, 5 / 13)
+        s = green.scale(a, alpha, C0)
+        solver = pcg.ResidentPCG(a, C0, np.eye(6), batch=6, s=s)
+        f = np.concatenate([solver.rhs(m) for m in np.eye(6)])
+        t0 = time.perf_counter()
+        u = hex8.project(np.zeros_like(f))
+        r = f - solver.true_K(u)
+        t_cert = (time.perf_counter() - t0) / 6
+        norm = np.sqrt((f * f).sum(1))
+        target2 = np.maximum(norm * 1e-8, 1e-13 / math.sqrt(a.shape[0])) ** 2
+        per_iter, sampled = [], 0
+        for _ in range(runs):
+            uu, rr = u.copy(), r.copy()
+            active = (rr * rr).sum(1) > target2
+            z = solver.B(rr)
+            p = np.where(active[:, None], z, 0.0)
+            rz = (rr * z).sum(1)
+            it = np.zeros(6, np.int64)
+            bad = np.zeros(6, bool)
+            t0 = time.perf_counter()
+            iters = 0
+            while time.perf_counter() - t0 < seconds / runs and active.any():
+                uu, rr, p, rz, active, it, bad = solver.iteration(uu, rr, p, rz, active, it, bad, target2)
+                iters += 1
+            per_iter.append((time.perf_counter() - t0) / max(int(it.sum()), 1))
+            sampled += int(it.sum())
+    finally:
+        green.FFT_WORKERS = None
+    t_load_iter = float(np.median(per_iter))
+    per_solve = t_load_iter * load_iterations + 12 * t_cert
+    blas = max([i.get('num_threads', 1) for i in threadpool_info()] or [1])
+    return {'value': 1.0 / per_solve, 'unit': 'certified 6-load solves/s', 'cores': int(max(workers, blas)),
+            'kind': 'port', 'host': host_info(),
+            'ms_per_load_iteration': 1e3 * t_load_iter, 'runs_ms_per_load_iteration': [1e3 * v for v in per_iter],
+            'sample': f'{runs} runs of the oracle resident PCG on the six-load group of the first timed design '
+                      f'step at {a.shape[0]}^3 ({sampled} load-iterations in all; numpy FP64, scipy.fft with '
+                      f'{workers} workers, elementwise numpy on one thread, BLAS {blas} threads), median '
+                      f'{t_load_iter * 1e3:.1f} ms per load-iteration, scaled to the timed GPU steps\' mean of '
+                      f'{load_iterations:.0f} load-iterations plus 12 certification applies'}
 
 
 def profile_stages(args):
@@ -172,11 +199,14 @@ def profile_stages(args):
     dst.copy_(src)
     torch.cuda.synchronize()
     n = args.n
-    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=args.batch).capture()
+    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=args.batch, streams=args.streams).capture()
     res.material(torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda'))
     res.select(args.alpha)
-    durations, active = probe_stages(res, args.probe_iters)
-    print(json.dumps({'stages_ms': durations, 'active': active, 'copy_bytes': 2 * src.numel() * 8}), flush=True)
+    parts = res._halves.parts
+    nl = parts[0][1] if parts else res.batch
+    durations, active = probe_stages(res, args.probe_iters, nl)
+    print(json.dumps({'stages_ms': durations, 'active': active, 'copy_bytes': 2 * src.numel() * 8,
+                      'loads_per_launch': nl, 'set_loads': res.batch}), flush=True)
 
 
 def batched_form(args):
@@ -560,6 +590,84 @@ def design_steps_compare(args):
     return out
 
 
+def served_64(args):
+    """The 64^3 served stream against the plain drop-in on the same designs.
+
+    ``--served-designs`` clients at 64^3 (Emin 1e-6, radius-4 filtered noise of their own seeds, alpha 0.5),
+    each running warm-started Study-II design steps (volume 0.35, move 0.2). Served form: the continuous-batching
+    server with every client in one resident set of FP32-inner / FP64-refinement slots, certified in FP64 at
+    1e-8. Plain: the two-stream FP64 drop-in serving the same requests one at a time with the same warm starts.
+    Each path first runs one cold (zero-start) step per client, untimed; then ``--served-steps`` requests per
+    client all waiting at time 0 are timed (closed loop: throughput). Last, Poisson arrivals at 60 % of the
+    served form's measured rate give its p50/p99 request latency under a sustainable load.
+    """
+    from pgj.serve import SequentialServer, Server, latency_stats, Client, poisson_arrivals
+
+    D, S = args.served_designs, args.served_steps
+    specs = [(64, args.emin, args.alpha, args.seed + 100 + d) for d in range(D)]
+
+    def clients(steps):
+        return [Client(i, n, emin, alpha, seed, steps, radius=4) for i, (n, emin, alpha, seed) in enumerate(specs)]
+
+    warm = [(0.0, i) for i in range(D)]
+    timed = [(0.0, i) for i in range(D) for _ in range(S)]
+    # served: one size class holding all D designs
+    served_clients = clients(1 + S)
+    server = Server((64,), 1e12, max_slots=D, precision='mixed', volume=args.volume, move=args.move,
+                    rmin=args.rmin)
+    server.run(served_clients, warm)
+    torch.cuda.synchronize()
+    done = server.run(served_clients, timed)
+    t_served = server.elapsed
+    served_rate = len(done) / t_served
+    served_failed = len(server.failed)
+    phases = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in server.phases.items()}
+    # plain: the same requests through the drop-in, one at a time
+    plain_clients = clients(1 + S)
+    base = SequentialServer((64,), volume=args.volume, move=args.move, rmin=args.rmin)
+    base.run(plain_clients, warm)
+    torch.cuda.synchronize()
+    base_done = base.run(plain_clients, timed)
+    t_plain = base.elapsed
+    plain_rate = len(base_done) / t_plain
+    # C^H of each timed step, served vs plain (the designs follow the same trajectory to solver precision)
+    gap = 0.0
+    for a, b in zip(served_clients, plain_clients):
+        for ha, hb in zip(a.history[1:], b.history[1:]):
+            gap = max(gap, float(np.abs(ha['CH'] - hb['CH']).max() / np.abs(hb['CH']).max()))
+    rel = max(h['max_rel'] for c in served_clients for h in c.history)
+    # latency under a sustainable load: Poisson arrivals at 60 % of the closed-loop served rate
+    rate = 0.6 * served_rate
+    lat_clients = clients(10 ** 6)
+    window = 6.0
+    arrivals = poisson_arrivals(rate, window, [c.steps for c in lat_clients], args.seed)
+    del server
+    torch.cuda.empty_cache()
+    lat_server = Server((64,), 1e12, max_slots=D, precision='mixed', volume=args.volume, move=args.move,
+                        rmin=args.rmin)
+    lat_done = lat_server.run(lat_clients, arrivals, horizon=window, deadline=3 * window)
+    stats = latency_stats(lat_done)
+    return {
+        'workload': f'{D} clients at 64^3, Emin {args.emin:g}, alpha {args.alpha:g}, radius-4 filtered noise '
+                    f'(seeds {args.seed + 100}..{args.seed + 99 + D}), one cold step each untimed, then {S} '
+                    f'warm-started Study-II design steps each (volume {args.volume:g}, move {args.move:g}), '
+                    f'all requests waiting at t = 0',
+        'served_mixed': {'value': served_rate, 'unit': 'certified design steps/s', 'seconds': t_served,
+                         'completed': len(done), 'failed_certification': served_failed,
+                         'max_relative_residual': rel,
+                         'phases_s': phases},
+        'plain_fp64': {'value': plain_rate, 'unit': 'certified design steps/s', 'seconds': t_plain,
+                       'completed': len(base_done), 'failed_certification': len(base.failed)},
+        'speedup_vs_plain': served_rate / plain_rate,
+        'ch_gap_vs_plain_max_rel': gap,
+        'serve_latency': {'arrival_rate_per_s': rate, 'window_s': window, 'completed': stats['count'],
+                          'note': 'new clients start cold (zero warm start); each client has one request in '
+                                  'flight at a time',
+                          'p50_ms': 1e3 * stats['p50'], 'p99_ms': 1e3 * stats['p99'],
+                          'unserved': lat_server.unserved},
+    }
+
+
 def serve_form(args):
     """Poisson design-step requests from many clients through the continuous-batching server."""
     from pgj.serve import SequentialServer, Server, latency_stats, make_clients, poisson_arrivals
@@ -633,6 +741,10 @@ def main():
     if args.form == 'configs':
         torch.cuda.set_device(0)
         return configs_form(args)
+    if args.form == 'served64':
+        torch.cuda.set_device(0)
+        print(json.dumps(served_64(args)), flush=True)
+        return
     if args.profile_stages:
         torch.cuda.set_device(0)
         return profile_stages(args)
@@ -644,7 +756,11 @@ def main():
     n = args.n
     rho0 = filtered_noise(n, args.seed, args.radius)
     res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel' if args.batch > 1 else 'serial',
-                       batch=args.batch if args.batch > 1 else None, streams=args.streams).capture()
+                       batch=args.batch if args.batch > 1 else None, streams=args.streams)
+    res.material(torch.tensor(filtered_noise(n, args.seed, args.radius), device='cuda') ** 3)
+    res.select(args.alpha)
+    tuned = res.tune() if args.tune else None       # row-march shape and FFT radix plans, before capture
+    res.capture()
     outer = PaperOuter(n, args.volume, rmin=args.rmin, beta=4.0, eta=0.5, emin=args.emin, move=args.move).capture()
     outer.set_initial(torch.tensor(rho0, device='cuda'))
     log = []
@@ -653,21 +769,25 @@ def main():
         outer.evaluate()
         res.material(outer.a)
         res.select(args.alpha)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         infos, checks = res.solve(rtol=1e-8)
+        torch.cuda.synchronize()
+        t_solve = time.perf_counter() - t0
         obj, local, _ = res.objective_energy('shear')
         H = res.effective_tensor()
         count = outer.step(local)
         outer.advance()
         log.append(dict(iterations=[i['iterations'] for i in infos], restarts=max(i['restarts'] for i in infos),
-                        rel=max(i['relative_residual'] for i in infos), checks=checks, bisections=count))
+                        rel=max(i['relative_residual'] for i in infos), checks=checks, bisections=count,
+                        solve_s=t_solve))
         return H
 
-    first_a = None
     for k in range(args.warmup):
         step()
-        if first_a is None:
-            first_a = res.op._material_field.cpu().numpy()
     torch.cuda.synchronize()
+    outer.evaluate()                                    # idempotent: the SIMP field of the first timed step
+    first_a = outer.a.cpu().numpy()
     lat = []
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -685,16 +805,30 @@ def main():
                        for j in range(6)], axis=1)
     ch_err = float(np.abs(stress - Hn).max() / np.abs(Hn).max())
 
-    durations, active = probe_stages(res, args.probe_iters)
-    nbytes = stage_bytes(res.batch, n)
-    stages = {k: {'ms': durations[k], 'bytes': nbytes[k], 'GBs': nbytes[k] / durations[k] / 1e6} for k in durations}
-    dom = max(durations, key=durations.get)
-    achieved = nbytes[dom] / durations[dom] / 1e6
-    its_first = sum(log[0]['iterations'])
+    # the solve's launch shape: each launch covers one slice of the group (two 3-load halves by default)
+    parts = res._halves.parts
+    nl = parts[0][1] if parts else res.batch
+    durations, active = probe_stages(res, args.probe_iters, nl)
+    nbytes = stage_bytes(nl, n)
+    stages = {k: {'ms': durations[k], 'bytes': nbytes[k], 'GBs': nbytes[k] / durations[k] / 1e6,
+                  'frac': nbytes[k] / durations[k] / 1e6 / HBM_PEAK_GBS} for k in durations}
+    # the four stages run once per iteration each and take 20-23 % of the solve's device time apiece (kernel
+    # trace), so the roofline reports the one furthest below the HBM peak
+    dom = min(stages, key=lambda k: stages[k]['GBs'])
+    achieved = stages[dom]['GBs']
+    load_its = sum(sum(r['iterations']) for r in timed)
+    solve_s = sum(r['solve_s'] for r in timed)
+    per_load_it = sum(nbytes.values()) / nl            # algorithmic bytes per load-iteration at this shape
+    aggregate = {'us_per_load_iteration': 1e6 * solve_s / load_its, 'solve_fraction_of_step': solve_s / elapsed,
+                 'algorithmic_bytes_per_load_iteration': per_load_it,
+                 'GBs': per_load_it * load_its / solve_s / 1e9,
+                 'frac': per_load_it * load_its / solve_s / 1e9 / HBM_PEAK_GBS,
+                 'note': 'all four stages of every load-iteration of the timed solves (two streams overlapping, '
+                         'host checks and certification included) over the solve wall time'}
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(first_a if first_a is not None else res.op._material_field.cpu().numpy(), args.alpha,
-                           its_first, args.cpu_seconds)
+        cpu = cpu_baseline(first_a, args.alpha, load_its / len(timed), args.cpu_seconds)
+    forms = None if args.no_forms else served_64(args)
     out = {
         'metric': 'certified 6-load solves/s at 64^3 (res <=1e-8, C^H err) + p50/p99 serve latency',
         'value': value, 'unit': 'certified 6-load solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
@@ -713,9 +847,12 @@ def main():
         'ch_stress_vs_energy_rel': ch_err,
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS,
-                     'traffic': MEASURED_TRAFFIC_64_6[dom] if (n, res.batch) == (64, 6) else None},
-        'stages': stages, 'probe_active': active,
+                     'traffic': MEASURED_TRAFFIC.get((n, res.batch, nl), {}).get(dom),
+                     'launch': f'{nl} of the {res.batch} loads of the set per launch (the timed solve\'s slices)'},
+        'stages': stages, 'probe_active': active, 'tuned': tuned,
+        'solve_aggregate': aggregate,
         'cpu_baseline': cpu,
+        'forms': forms,
     }
     print(json.dumps(out), flush=True)
 

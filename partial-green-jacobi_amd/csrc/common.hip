@@ -31,6 +31,18 @@ std::mutex g_shape_mutex;
 std::map<ShapeKey, std::tuple<int, int, int, int>> g_shapes;
 }  // namespace
 
+namespace {
+using PlanKey = std::tuple<int, int, int>;  // cube n, element bytes, pass (0 forward xy, 1 z, 2 inverse xy)
+std::mutex g_plan_mutex;
+std::map<PlanKey, int> g_plans;
+}  // namespace
+
+int fft_plan(int n, int es, int pass) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  auto it = g_plans.find(PlanKey(n, es, pass));
+  return it == g_plans.end() ? 0 : it->second;
+}
+
 bool tuned_shape(const pgj_grid& g, int L, int es, int* R, int* W, int* ZC, int* plane) {
   std::lock_guard<std::mutex> lock(g_shape_mutex);
   auto it = g_shapes.find(ShapeKey(g.nz, g.ny, g.nx, L, es));
@@ -61,6 +73,21 @@ extern "C" int pgj_hex8_get_shape(const pgj_grid* g, int L, int es, int* out5) {
   const pgj::RowMarch c = pgj::rowmarch_config(*g, L, es);
   PGJ_REQUIRE(c.valid, "grid has no row-march shape");
   out5[0] = c.R; out5[1] = c.W; out5[2] = c.ZC; out5[3] = c.plane; out5[4] = c.by * c.bz;
+  return PGJ_OK;
+}
+
+extern "C" int pgj_fft_set_plan(const pgj_grid* g, int es, int pass, int plan) {
+  PGJ_REQUIRE(g && g->nx == g->ny && g->ny == g->nz && (es == 4 || es == 8) && pass >= 0 && pass <= 2 &&
+                  (plan == 0 || plan == 1),
+              "FFT plan needs a cube, es in {4,8}, pass in 0..2 and plan in {0,1}");
+  std::lock_guard<std::mutex> lock(pgj::g_plan_mutex);
+  pgj::g_plans[pgj::PlanKey(g->nx, es, pass)] = plan;
+  return PGJ_OK;
+}
+
+extern "C" int pgj_fft_get_plan(const pgj_grid* g, int es, int* out3) {
+  PGJ_REQUIRE(g && (es == 4 || es == 8) && out3, "invalid FFT plan query");
+  for (int pass = 0; pass < 3; ++pass) out3[pass] = pgj::fft_plan(g->nx, es, pass);
   return PGJ_OK;
 }
 

@@ -122,7 +122,7 @@ __device__ inline int pair_of(int i) {
 }
 
 // ------------------------------------------------------------ forward xy
-template <int N, int MODE, typename R>
+template <int N, int MODE, typename R, int PLAN = 0>
 __global__ __launch_bounds__(Cube<N>::TXY) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_xy<R>())))
 void fxy_static(FFTArgsT<R> A) {
   using C = Cube<N>;
@@ -216,7 +216,7 @@ void fxy_static(FFTArgsT<R> A) {
     }
   }
   __syncthreads();
-  sfft<false, C::M, typename C::Rows, T>(plane, tw, 2);
+  sfft<false, C::M, typename C::Rows, T, PLAN>(plane, tw, 2);
   // split the packed spectrum of every row into its half spectrum
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
   for (int b = threadIdx.x; b < PAIRS; b += T) {
@@ -230,13 +230,13 @@ void fxy_static(FFTArgsT<R> A) {
     if (C::M - k != k) L[C::M - k] = conjd(csub(E, wo));
   }
   __syncthreads();
-  sfft<false, N, typename C::Cols, T>(plane, tw, 1);
+  sfft<false, N, typename C::Cols, T, PLAN>(plane, tw, 1);
   X* out = A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX;
   for (int e = threadIdx.x; e < N * C::NKX; e += T) out[e] = plane[e];
 }
 
 // ---------------------------------------------------------------- z + Green
-template <int N, int MODE, typename R, int KCX = 0>
+template <int N, int MODE, typename R, int KCX = 0, int PLAN = 0>
 __global__ __launch_bounds__((ZPass<N, R, KCX>::TZ)) __attribute__((amdgpu_waves_per_eu((ZPass<N, R, KCX>::occ))))
 void fz_static(FFTArgsT<R> A) {
   using C = Cube<N>;
@@ -364,7 +364,7 @@ void fz_static(FFTArgsT<R> A) {
   };
   if (MODE != kFzScale && EARLY) load_group(0);
   __syncthreads();
-  sfft<false, N, typename Z::ZCols, T>(cols, tw, 1);
+  sfft<false, N, typename Z::ZCols, T, PLAN>(cols, tw, 1);
   if (MODE == kFzScale) {
     for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
       const int kcol = e % KC, cz = e / KC;
@@ -414,7 +414,7 @@ void fz_static(FFTArgsT<R> A) {
     }
   }
   __syncthreads();
-  sfft<true, N, typename Z::ZCols, T>(cols, tw, 1);
+  sfft<true, N, typename Z::ZCols, T, PLAN>(cols, tw, 1);
   for (int e = threadIdx.x; e < 3 * N * KC; e += T) {
     const int kcol = e % KC, cz = e / KC;
     if (Z::KFULL || kcol < kcn) S[(int64_t(cz) * N + y) * NKX + kx0 + kcol] = cols[e];
@@ -433,7 +433,7 @@ void fz_static(FFTArgsT<R> A) {
 
 // --------------------------------------------------------------- inverse xy
 // MODE kFxyiPre: the preconditioner apply alone (z = P(s w), no PCG state).
-template <int N, int MODE, typename R>
+template <int N, int MODE, typename R, int PLAN = 0>
 __global__ __launch_bounds__(Cube<N>::TXY) __attribute__((amdgpu_waves_per_eu(Cube<N>::template occ_xy<R>())))
 void fxyi_static(FFTArgsT<R> A) {
   using C = Cube<N>;
@@ -525,7 +525,7 @@ void fxyi_static(FFTArgsT<R> A) {
   };
   if (EARLY) load_chunk(0);
   __syncthreads();
-  sfft<true, N, typename C::Cols, T>(plane, tw, 1);
+  sfft<true, N, typename C::Cols, T, PLAN>(plane, tw, 1);
   constexpr int HALF = C::M / 2, PAIRS = N * (HALF + 1);
   for (int b = threadIdx.x; b < PAIRS; b += T) {
     const int row = b / (HALF + 1), k = b % (HALF + 1);
@@ -548,7 +548,7 @@ void fxyi_static(FFTArgsT<R> A) {
     L[k] = zk;
   }
   __syncthreads();
-  sfft<true, C::M, typename C::Rows, T>(plane, tw, 2);
+  sfft<true, C::M, typename C::Rows, T, PLAN>(plane, tw, 2);
 #pragma unroll
   for (int i0 = 0; i0 < C::ITER; i0 += C::CH) {
     if (!(EARLY && i0 == 0)) load_chunk(i0);
@@ -595,8 +595,13 @@ int fxy_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
       hipLaunchKernelGGL((fxy_static<N, kFxyPre, R>), grid, block, lds, s, A);
       break;
     default:
-      if (int e = allow_lds_static(fxy_static<N, kFxyPCG, R>, lds)) return e;
-      hipLaunchKernelGGL((fxy_static<N, kFxyPCG, R>), grid, block, lds, s, A);
+      if (fft_plan(N, int(sizeof(R)), 0) == 1) {
+        if (int e = allow_lds_static(fxy_static<N, kFxyPCG, R, 1>, lds)) return e;
+        hipLaunchKernelGGL((fxy_static<N, kFxyPCG, R, 1>), grid, block, lds, s, A);
+      } else {
+        if (int e = allow_lds_static(fxy_static<N, kFxyPCG, R>, lds)) return e;
+        hipLaunchKernelGGL((fxy_static<N, kFxyPCG, R>), grid, block, lds, s, A);
+      }
   }
   PGJ_LAUNCHED("fxy_static");
   return PGJ_OK;
@@ -623,8 +628,13 @@ int fz_static_kc(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
       hipLaunchKernelGGL((fz_static<N, kFzInit, R, KCX>), grid, block, lds, s, A);
       break;
     default:
-      if (int e = allow_lds_static(fz_static<N, kFzPCG, R, KCX>, lds)) return e;
-      hipLaunchKernelGGL((fz_static<N, kFzPCG, R, KCX>), grid, block, lds, s, A);
+      if (fft_plan(N, int(sizeof(R)), 1) == 1) {
+        if (int e = allow_lds_static(fz_static<N, kFzPCG, R, KCX, 1>, lds)) return e;
+        hipLaunchKernelGGL((fz_static<N, kFzPCG, R, KCX, 1>), grid, block, lds, s, A);
+      } else {
+        if (int e = allow_lds_static(fz_static<N, kFzPCG, R, KCX>, lds)) return e;
+        hipLaunchKernelGGL((fz_static<N, kFzPCG, R, KCX>), grid, block, lds, s, A);
+      }
   }
   PGJ_LAUNCHED("fz_static");
   return PGJ_OK;
@@ -666,6 +676,9 @@ int fxyi_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   if (mode == kFxyiPre) {
     if (int e = allow_lds_static(fxyi_static<N, kFxyiPre, R>, lds)) return e;
     hipLaunchKernelGGL((fxyi_static<N, kFxyiPre, R>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);
+  } else if (fft_plan(N, int(sizeof(R)), 2) == 1) {
+    if (int e = allow_lds_static(fxyi_static<N, kFxyiPCG, R, 1>, lds)) return e;
+    hipLaunchKernelGGL((fxyi_static<N, kFxyiPCG, R, 1>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);
   } else {
     if (int e = allow_lds_static(fxyi_static<N, kFxyiPCG, R>, lds)) return e;
     hipLaunchKernelGGL((fxyi_static<N, kFxyiPCG, R>), dim3(N, 3, L), dim3(C::TXY), lds, s, A);

@@ -36,6 +36,11 @@ struct RowMarch {
 // Returns false if unset.
 bool tuned_shape(const pgj_grid& g, int L, int es, int* R, int* W, int* ZC, int* plane);
 
+// Radix plan of the compile-time FFT pass `pass` (0 forward xy, 1 z with the Green multiply, 2 inverse xy) of
+// the PCG iteration on an n^3 cube in precision es: 0 (radix 4 first, the default) or 1 (radix 8 first); set by
+// pgj_fft_set_plan (the autotuner) and read at every launch of that pass.
+int fft_plan(int n, int es, int pass);
+
 // Shape preferences (R, W, ZC) for L loads per launch. ZC (planes marched by
 // one block) is the largest of 16, 8, 4, 2, 1 dividing nz that still gives
 // at least 192 blocks: longer marches re-read fewer halo planes, but the

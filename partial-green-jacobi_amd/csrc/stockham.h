@@ -39,6 +39,28 @@ __device__ __forceinline__ void dft(C (&v)[R], const C* tw, int n, int step) {
     v[2] = csub(t0, t2);
     v[1] = cadd(t1, t3);
     v[3] = csub(t1, t3);
+  } else if constexpr (R == 8) {
+    // split radix-8: a_j = x_j + x_{j+4}, b_j = (x_j - x_{j+4}) W_8^j (conjugate for the inverse); the even
+    // outputs are the 4-point DFT of a, the odd ones that of b
+    constexpr T h = T(0.70710678118654752440);  // 1/sqrt(2)
+    C a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = cadd(v[j], v[j + 4]);
+      b[j] = csub(v[j], v[j + 4]);
+    }
+    // W_8^1 = (1 - i)/sqrt(2), W_8^2 = -i, W_8^3 = (-1 - i)/sqrt(2)
+    b[1] = INV ? c2(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y)) : c2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+    b[2] = INV ? c2(-b[2].y, b[2].x) : c2(b[2].y, -b[2].x);
+    b[3] = INV ? c2(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y))
+               : c2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+    dft<4, INV>(a, tw, n, step);
+    dft<4, INV>(b, tw, n, step);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      v[2 * m] = a[m];
+      v[2 * m + 1] = b[m];
+    }
   } else if constexpr (R == 3) {
     constexpr T h = T(0.86602540378443864676);  // sqrt(3)/2
     const C s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
@@ -61,9 +83,12 @@ __device__ __forceinline__ void dft(C (&v)[R], const C* tw, int n, int step) {
   }
 }
 
-// Radix plan of a length: radices 4 first, then 2, 3, 5, 7 (same order as factorize()).
-constexpr int radix_count(int n) {
+// Radix plans of a length. Plan 0: radices 4 first, then 2, 3, 5, 7 (the order of factorize()). Plan 1:
+// radix 8 first, then as plan 0 (64 = 8 x 8: two LDS passes instead of three).
+constexpr int radix_count(int n, int plan = 0) {
   int c = 0;
+  if (plan == 1)
+    while (n % 8 == 0) { n /= 8; ++c; }
   while (n % 4 == 0) { n /= 4; ++c; }
   while (n % 2 == 0) { n /= 2; ++c; }
   for (int p = 3; p <= 7; p += 2)
@@ -71,8 +96,10 @@ constexpr int radix_count(int n) {
   return c;
 }
 
-constexpr int radix_at(int n, int s) {
+constexpr int radix_at(int n, int s, int plan = 0) {
   int c = 0;
+  if (plan == 1)
+    while (n % 8 == 0) { if (c++ == s) return 8; n /= 8; }
   while (n % 4 == 0) { if (c++ == s) return 4; n /= 4; }
   while (n % 2 == 0) { if (c++ == s) return 2; n /= 2; }
   for (int p = 3; p <= 7; p += 2)
@@ -80,9 +107,9 @@ constexpr int radix_at(int n, int s) {
   return 1;
 }
 
-constexpr int radix_ns(int n, int s) {
+constexpr int radix_ns(int n, int s, int plan = 0) {
   int prod = 1;
-  for (int k = 0; k < s; ++k) prod *= radix_at(n, k);
+  for (int k = 0; k < s; ++k) prod *= radix_at(n, k, plan);
   return prod;
 }
 
@@ -134,11 +161,11 @@ __device__ __forceinline__ void sstage(C* buf, const C* __restrict__ tw, int ste
   __syncthreads();
 }
 
-template <bool INV, int N, class LN, int THREADS, int S = 0, typename C>
+template <bool INV, int N, class LN, int THREADS, int PLAN = 0, int S = 0, typename C>
 __device__ __forceinline__ void sfft(C* buf, const C* __restrict__ tw, int step) {
-  if constexpr (S < radix_count(N)) {
-    sstage<radix_at(N, S), INV, N, radix_ns(N, S), LN, THREADS>(buf, tw, step);
-    sfft<INV, N, LN, THREADS, S + 1>(buf, tw, step);
+  if constexpr (S < radix_count(N, PLAN)) {
+    sstage<radix_at(N, S, PLAN), INV, N, radix_ns(N, S, PLAN), LN, THREADS>(buf, tw, step);
+    sfft<INV, N, LN, THREADS, PLAN, S + 1>(buf, tw, step);
   }
 }
 

@@ -174,38 +174,23 @@ class BatchedDesigns:
             fork = self._forks[name] = Fork(len(groups) - 1)
         return lambda: fork.run([getattr(g, name) for g in groups])
 
-    def tune(self, reps=6, candidates=None):
-        """Time row-march shapes per group on its own buffers and keep the fastest; returns the choices."""
+    def tune(self, reps=6, candidates=None, fft=True):
+        """Time row-march shapes and FFT radix plans per group on its own buffers and keep the fastest; returns
+        the choices (``pgj.tune``)."""
+        from . import tune as tuner
         chosen = {}
         for n, g in self.groups.items():
-            grid = g.op.grid
-            lib = _lib.lib()
-            if candidates is None:
-                # (R, W, ZC, plane); the FP64 plane-carry march runs at most 4 row waves per block
-                cand = [(1, w, zc, p) for p in (0, 1) for w in (2, 4, 8) for zc in (1, 2, 4, 8, 16)
-                        if n % zc == 0 and not (p and w > 4)]
-            else:
-                cand = list(candidates)
             g.setup(1e-8)
             g.r.normal_()
             g.certify()
             g.initialize()
             g.stage(0, g.p0, g.p1)
-            best = None
-            for shape in cand:
-                _lib.check(lib.pgj_hex8_set_shape(grid, g.L, 8, *shape))
-                g.stage(0, g.p0, g.p1)   # warm
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                for _ in range(reps):
-                    g.stage(0, g.p0, g.p1)
-                e1.record()
-                torch.cuda.synchronize()
-                t = e0.elapsed_time(e1) / reps
-                if best is None or t < best[0]:
-                    best = (t, shape)
-            _lib.check(lib.pgj_hex8_set_shape(grid, g.L, 8, *best[1]))
-            chosen[n] = dict(shape=best[1], ms=best[0])
+            row = tuner.tune_row_march(g.op.grid, g.L, 8, lambda: g.stage(0, g.p0, g.p1), reps, candidates)
+            chosen[n] = dict(shape=row['shape'], ms=row['ms'])
+            if fft:
+                for k in range(1, 4):
+                    g.stage(k, g.p0, g.p1)
+                chosen[n]['fft'] = tuner.tune_fft(g.op.grid, 8, lambda k: g.stage(k, g.p0, g.p1), reps)
             g.u.zero_()
         torch.cuda.synchronize()
         return chosen

@@ -150,6 +150,35 @@ class MixedResident:
         sl = slice(l0, l0 + nl)
         return torch.stack((self.norm[sl], self.absolute[sl], self.target2[sl].sqrt()), dim=1).cpu().numpy()
 
+    def tune(self, reps=6, fft=True):
+        """Pick the FP32 HEX8 row-march shape and the FP32 FFT radix plans by timing them on this set's buffers
+        at its launch shape; returns the choices (``pgj.tune``). Graphs captured before are captured again."""
+        from . import tune as tuner
+        parts = self._halves.parts
+        nl = parts[0][1] if parts else self.L
+        self.setup(1e-8)
+        self.u.normal_()
+        self.certify()
+        self.begin()
+
+        def stage(k):
+            _lib.call('pgj_inner_stage', self.op.grid, self.L, self.G, 0, nl, _lib.ptr(self.plan32),
+                      _lib.ptr(self.a32), _lib.ptr(self.scale32), _lib.ptr(self.scale_hat32), _lib.ptr(self.d),
+                      _lib.ptr(self.r32), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
+                      _lib.ptr(self.state32), _lib.ptr(self.ws32), k, _lib.stream())
+
+        for k in range(4):
+            stage(k)
+        out = {'row_march': tuner.tune_row_march(self.op.grid, nl, 4, lambda: stage(0), reps)}
+        if fft:
+            out['fft'] = tuner.tune_fft(self.op.grid, 4, stage, reps)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.active32.zero_()
+        if self.graphs is not None:
+            self.capture()
+        return out
+
     def capture(self):
         self.setup(1e-8)
         self.u.normal_()

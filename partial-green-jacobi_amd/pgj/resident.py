@@ -183,6 +183,36 @@ class Resident:
         count = int(count or self.chunk)
         self._halves.run(lambda l0, nl: self._iterate_slice(l0, nl, count), self.batch)
 
+    def tune(self, reps=6, fft=True):
+        """Pick the HEX8 row-march shape and the FFT radix plans by timing them on this solver's buffers at the
+        solve's launch shape (one slice of the group per launch); returns the choices (``pgj.tune``). The
+        choices hold for every solver of this grid; graphs captured before are captured again."""
+        from . import tune as tuner
+        parts = self._halves.parts
+        nl = parts[0][1] if parts else self.batch
+        self._setup(range(self.batch), 1e-8)
+        self.r.normal_()
+        self.certify()
+        self.initialize()
+
+        def stage(k):
+            _lib.call('pgj_resident_stage', self.op.grid, self.batch, self.batch, 0, nl, _lib.ptr(self.op._plan),
+                      _lib.ptr(self.op._material_field), _lib.ptr(self.scale), _lib.ptr(self.scale_hat),
+                      _lib.ptr(self.u), _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.p1), _lib.ptr(self.q),
+                      _lib.ptr(self.state), _lib.ptr(self.ws), k, _lib.stream())
+
+        for k in range(4):
+            stage(k)
+        out = {'row_march': tuner.tune_row_march(self.op.grid, nl, 8, lambda: stage(0), reps)}
+        if fft and isinstance(self.n, int):
+            out['fft'] = tuner.tune_fft(self.op.grid, 8, stage, reps)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.solutions.zero_()
+        if self.iter_graph is not None:
+            self.capture()
+        return out
+
     def capture(self):
         """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``), and the outer loop's."""
         if self.outer is not None:

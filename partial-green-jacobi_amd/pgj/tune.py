@@ -1,0 +1,75 @@
+"""Launch-shape autotuner: HEX8 row-march shapes and FFT radix plans, timed on a set's own buffers.
+
+Counterpart of the reference's variant and tile sweeps (``solver/src/hex8_modal/gpu_study.py:174-222``,
+``modal_adapter.py:28-41``). Every candidate computes the same operator, so a choice changes only speed and
+rounding. A choice is recorded in the library (``pgj_hex8_set_shape``, ``pgj_fft_set_plan``) and holds for later
+launches and graph captures of that grid; tune before capturing.
+"""
+import torch
+
+from . import _lib
+
+STAGE_FXY, STAGE_FZ, STAGE_FXYI = 1, 2, 3
+
+
+def time_launch(fn, reps):
+    """Mean device milliseconds of ``fn()`` over ``reps`` back-to-back launches after one warm launch."""
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def row_march_candidates(n, es):
+    """(R, W, ZC, plane) shapes; the FP64 plane-carry march runs at most 4 row waves per block."""
+    return [(1, w, zc, p) for p in (0, 1) for w in (2, 4, 8) for zc in (1, 2, 4, 8, 16)
+            if n % zc == 0 and not (p and w > 4 and es == 8)]
+
+
+def tune_row_march(grid, L, es, launch_hex8, reps=6, candidates=None):
+    """Fastest row-march shape of ``launch_hex8()`` (a HEX8 stage over L loads); recorded and returned."""
+    lib = _lib.lib()
+    best = None
+    for shape in candidates or row_march_candidates(grid.nx, es):
+        _lib.check(lib.pgj_hex8_set_shape(grid, L, es, *shape))
+        t = time_launch(launch_hex8, reps)
+        if best is None or t < best[0]:
+            best = (t, shape)
+    _lib.check(lib.pgj_hex8_set_shape(grid, L, es, *best[1]))
+    return dict(shape=best[1], ms=best[0])
+
+
+def tune_fft(grid, es, launch_stage, reps=6):
+    """Radix plan (0: radix 4 first, 1: radix 8 first) of each FFT pass, by timing ``launch_stage(k)`` for
+    stages 1 (forward xy), 2 (z and Green multiply) and 3 (inverse xy); recorded and returned."""
+    lib = _lib.lib()
+    out = {}
+    for pas, stage in enumerate((STAGE_FXY, STAGE_FZ, STAGE_FXYI)):
+        times = {}
+        for plan in (0, 1):
+            _lib.check(lib.pgj_fft_set_plan(grid, es, pas, plan))
+            times[plan] = time_launch(lambda: launch_stage(stage), reps)
+        plan = min(times, key=times.get)
+        _lib.check(lib.pgj_fft_set_plan(grid, es, pas, plan))
+        out[('fxy', 'fz', 'fxyi')[pas]] = dict(plan=plan, ms={str(k): v for k, v in times.items()})
+    return out
+
+
+def fft_plans(grid, es):
+    """The radix plans in use for the three passes of a cube in precision es."""
+    out = (_lib.c_int * 3)()
+    _lib.check(_lib.lib().pgj_fft_get_plan(grid, es, out))
+    return tuple(out)
+
+
+def reset(grid, L=None, es=8):
+    """Default shapes and plans again (row-march shape for L loads per launch, when given)."""
+    lib = _lib.lib()
+    for pas in range(3):
+        _lib.check(lib.pgj_fft_set_plan(grid, es, pas, 0))
+    if L is not None:
+        _lib.check(lib.pgj_hex8_set_shape(grid, L, es, 0, 0, 0, 0))

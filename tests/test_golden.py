@@ -64,3 +64,18 @@ def test_oc_fixture():
 def test_fixtures_hold_plain_arrays(name):
     g = load(name)
     assert all(v.dtype.kind in 'fiu' for v in g.values())
+
+
+def test_solve64_fixture_is_consistent():
+    """The 64^3 oracle solves (minutes of CPU to rerun): certified, C^H symmetric, SPD and the same at every
+    alpha (the preconditioner changes the iterates, not the solution)."""
+    g = load('solve_64_alpha')
+    H = g['CH_a05']
+    for tag in ('a0', 'a05', 'a1'):
+        assert np.all(g['relative_residual_' + tag] <= 1e-8)
+        assert g['iterations_' + tag].shape == (6,)
+        np.testing.assert_allclose(g['CH_' + tag], H, rtol=0, atol=1e-9 * np.abs(H).max())
+    assert np.abs(H - H.T).max() <= 1e-12 * np.abs(H).max()
+    assert np.linalg.eigvalsh(H).min() > 0
+    # Green (alpha 0) needs several times the iterations of the scaled preconditioners at this contrast
+    assert g['iterations_a0'].min() > 4 * g['iterations_a05'].max()
