@@ -614,7 +614,7 @@ def served_64(args):
     # served: one size class holding all D designs
     served_clients = clients(1 + S)
     server = Server((64,), 1e12, max_slots=D, precision='mixed', volume=args.volume, move=args.move,
-                    rmin=args.rmin)
+                    rmin=args.rmin, tune=bool(args.tune))
     server.run(served_clients, warm)
     torch.cuda.synchronize()
     done = server.run(served_clients, timed)
@@ -622,9 +622,10 @@ def served_64(args):
     served_rate = len(done) / t_served
     served_failed = len(server.failed)
     phases = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in server.phases.items()}
+    server_tuned = server.tuned
     # plain: the same requests through the drop-in, one at a time
     plain_clients = clients(1 + S)
-    base = SequentialServer((64,), volume=args.volume, move=args.move, rmin=args.rmin)
+    base = SequentialServer((64,), volume=args.volume, move=args.move, rmin=args.rmin, tune=bool(args.tune))
     base.run(plain_clients, warm)
     torch.cuda.synchronize()
     base_done = base.run(plain_clients, timed)
@@ -644,7 +645,7 @@ def served_64(args):
     del server
     torch.cuda.empty_cache()
     lat_server = Server((64,), 1e12, max_slots=D, precision='mixed', volume=args.volume, move=args.move,
-                        rmin=args.rmin)
+                        rmin=args.rmin)             # the tuned shapes and plans of the first server still hold
     lat_done = lat_server.run(lat_clients, arrivals, horizon=window, deadline=3 * window)
     stats = latency_stats(lat_done)
     return {
@@ -659,6 +660,7 @@ def served_64(args):
         'plain_fp64': {'value': plain_rate, 'unit': 'certified design steps/s', 'seconds': t_plain,
                        'completed': len(base_done), 'failed_certification': len(base.failed)},
         'speedup_vs_plain': served_rate / plain_rate,
+        'tuned': {'served': server_tuned, 'plain': base.tuned},
         'ch_gap_vs_plain_max_rel': gap,
         'serve_latency': {'arrival_rate_per_s': rate, 'window_s': window, 'completed': stats['count'],
                           'note': 'new clients start cold (zero warm start); each client has one request in '
@@ -682,6 +684,7 @@ def serve_form(args):
     elapsed = server.elapsed
     server_unserved, server_slots = server.unserved, {str(n): g.D for n, g in server.groups.items()}
     phases = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in server.phases.items()}
+    server_tuned = server.tuned
     resident = sum(server.memory().values()) / 2 ** 30
     del server
     torch.cuda.empty_cache()

@@ -233,6 +233,11 @@ int pgj_alpha_select(const pgj_grid* g, int method, const double* params, const 
  * passes' block shapes, and so the layout of the partial sums one pass leaves
  * for the next, follow it. */
 
+/* 1 if the z pass of a resident set of L loads (its workspace's load count) with
+ * elements of es bytes runs on the wide kx chunks on the current device (64^3
+ * FP64 where the default chunks need more than one round of blocks), else 0. */
+int pgj_zpass_wide(const pgj_grid* g, int L, int es);
+
 /* rhs_l for the L macro strains (host L*6), norms, targets
  * max(rtol |f|, 1e-13/sqrt(nx))^2 (NaN if |f| is NaN, as torch.maximum),
  * iterations = 0, bad = 0, and the per-load iteration cap `cap` (>= 1; the

@@ -669,6 +669,14 @@ bool zpass_wide(const pgj_grid& g, int L, int es) {
   return cus > 0 && narrow > cus * Zn::per_cu;
 }
 
+}  // namespace pgj
+
+extern "C" int pgj_zpass_wide(const pgj_grid* g, int L, int es) {
+  return g && L >= 1 && pgj::zpass_wide(*g, L, es) ? 1 : 0;
+}
+
+namespace pgj {
+
 template <int N, typename R>
 int fxyi_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   using C = Cube<N>;

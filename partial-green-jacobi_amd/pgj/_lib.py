@@ -45,6 +45,7 @@ SIGNATURES = {
     'pgj_hex8_set_shape': (c_int, [GP, c_int, c_int, c_int, c_int, c_int, c_int]),
     'pgj_hex8_get_shape': (c_int, [GP, c_int, c_int, ctypes.POINTER(c_int)]),
     'pgj_fft_set_plan': (c_int, [GP, c_int, c_int, c_int]),
+    'pgj_zpass_wide': (c_int, [GP, c_int, c_int]),
     'pgj_fft_get_plan': (c_int, [GP, c_int, ctypes.POINTER(c_int)]),
     'pgj_aos_to_soa': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p]),
     'pgj_soa_to_aos': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p]),

@@ -135,6 +135,25 @@ class DesignGroup:
                   _lib.ptr(out), _lib.stream())
         return out.reshape(6, 6)
 
+    def tune(self, reps=6, fft=True):
+        """Row-march shape for this set's launch size and FFT radix plans, timed on its own buffers
+        (``pgj.tune``); returns the choices. Call before capturing graphs of this set."""
+        from . import tune as tuner
+        self.setup(1e-8)
+        self.r.normal_()
+        self.certify()
+        self.initialize()
+        for k in range(4):
+            self.stage(k, self.p0, self.p1)
+        out = {'row_march': tuner.tune_row_march(self.op.grid, self.L, 8, lambda: self.stage(0, self.p0, self.p1),
+                                                 reps)}
+        if fft:
+            out['fft'] = tuner.tune_fft(self.op.grid, 8, lambda k: self.stage(k, self.p0, self.p1), reps)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.active.zero_()
+        return out
+
     def shape_info(self):
         """(R, W, ZC, plane, blocks per load) of the row-march launch of this set."""
         out = (_lib.c_int * 5)()
@@ -180,18 +199,20 @@ class BatchedDesigns:
         from . import tune as tuner
         chosen = {}
         for n, g in self.groups.items():
-            g.setup(1e-8)
-            g.r.normal_()
-            g.certify()
-            g.initialize()
-            g.stage(0, g.p0, g.p1)
-            row = tuner.tune_row_march(g.op.grid, g.L, 8, lambda: g.stage(0, g.p0, g.p1), reps, candidates)
-            chosen[n] = dict(shape=row['shape'], ms=row['ms'])
-            if fft:
-                for k in range(1, 4):
-                    g.stage(k, g.p0, g.p1)
-                chosen[n]['fft'] = tuner.tune_fft(g.op.grid, 8, lambda k: g.stage(k, g.p0, g.p1), reps)
-            g.u.zero_()
+            if candidates is None:
+                choice = g.tune(reps, fft)
+            else:
+                g.setup(1e-8)
+                g.r.normal_()
+                g.certify()
+                g.initialize()
+                g.stage(0, g.p0, g.p1)
+                choice = {'row_march': tuner.tune_row_march(g.op.grid, g.L, 8, lambda: g.stage(0, g.p0, g.p1), reps,
+                                                            candidates)}
+                g.u.zero_()
+            chosen[n] = dict(shape=choice['row_march']['shape'], ms=choice['row_march']['ms'])
+            if 'fft' in choice:
+                chosen[n]['fft'] = choice['fft']
         torch.cuda.synchronize()
         return chosen
 

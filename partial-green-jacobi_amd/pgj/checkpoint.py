@@ -22,18 +22,22 @@ def _host(v):
 
 
 def save_state(path, **fields):
-    """Write the given fields (tensors or arrays; names or their aliases) to ``path`` (``np.savez_compressed``)."""
-    out = {}
+    """Write the given fields (tensors or arrays) to ``path`` (``np.savez_compressed``) under the names given, so
+    a Study-II state saved with ``rho``/``u0`` stays readable by the reference's own readers
+    (``examples/replay_state.py:43-45`` read ``state['rho']`` and ``state['u0']``). A field given under both
+    its names is refused."""
+    out, canonical = {}, {}
     for key, value in fields.items():
         if value is None:
             continue
         name = ALIASES.get(key, key)
-        if name in out:
-            raise ValueError(f'{key!r} given twice (as {name!r})')
-        out[name] = _host(value)
-        if out[name].dtype == object:
+        if name in canonical:
+            raise ValueError(f'{key!r} given twice (as {canonical[name]!r} and {key!r})')
+        canonical[name] = key
+        out[key] = _host(value)
+        if out[key].dtype == object:
             raise ValueError(f'{key!r}: object arrays are not stored (the files are read without pickle)')
-    check_state(out)
+    check_state({ALIASES.get(k, k): v for k, v in out.items()})
     np.savez_compressed(path, **out)
 
 

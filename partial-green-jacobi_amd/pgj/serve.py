@@ -85,7 +85,7 @@ class Server:
     """Design-step server: ``sizes`` are the grid sizes served, ``budget`` the HBM bytes for resident sets."""
 
     def __init__(self, sizes, budget, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, chunk=4, max_slots=16,
-                 device='cuda', graphs=True, precision='fp64', inner_rtol=1e-4, max_outer=30, cap=CAP):
+                 device='cuda', graphs=True, precision='fp64', inner_rtol=1e-4, max_outer=30, cap=CAP, tune=False):
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.sizes = sorted(int(n) for n in sizes)
@@ -109,6 +109,8 @@ class Server:
         self.solving = {n: {} for n in self.sizes}          # slot -> Request
         self.workers = {}
         self.graphs = {}
+        # row-march shapes and FFT radix plans for each size class's launch size, before any capture
+        self.tuned = {n: g.tune(reps=3) for n, g in self.groups.items()} if tune else None
         if graphs:
             self._capture()
 
@@ -373,12 +375,15 @@ class SequentialServer:
     starts when the previous one has finished.
     """
 
-    def __init__(self, sizes, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, device='cuda', cap=CAP):
+    def __init__(self, sizes, volume=0.35, move=0.2, rmin=3.0, rtol=1e-8, device='cuda', cap=CAP, tune=False):
         from .resident import Resident
         self.rtol, self.device = rtol, device
         self.volume, self.move, self.rmin = volume, move, rmin
-        self.solvers = {int(n): Resident(int(n), loads=LOADS6, load_mode='parallel', batch=6, device=device, cap=cap).capture()
-                        for n in sizes}
+        self.solvers = {int(n): Resident(int(n), loads=LOADS6, load_mode='parallel', batch=6, device=device,
+                                         cap=cap) for n in sizes}
+        self.tuned = {n: r.tune(reps=3) for n, r in self.solvers.items()} if tune else None
+        for r in self.solvers.values():
+            r.capture()
         self.workers = {}
         self.warm = {}
         self.unserved = 0

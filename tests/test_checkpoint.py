@@ -33,6 +33,10 @@ def test_older_key_names(tmp_path):
     np.testing.assert_array_equal(back['previous'], s['previous'])
     checkpoint.save_state(tmp_path / 'new.npz', rho=s['physical'], u0=s['previous'])
     assert sorted(checkpoint.load_state(tmp_path / 'new.npz')) == ['physical', 'previous']
+    # the reference's Study-II readers index the old names (examples/replay_state.py:43-45)
+    with np.load(tmp_path / 'new.npz', allow_pickle=False) as z:
+        np.testing.assert_array_equal(z['rho'], s['physical'])
+        np.testing.assert_array_equal(z['u0'], s['previous'])
     with pytest.raises(ValueError):
         checkpoint.save_state(tmp_path / 'dup.npz', rho=s['physical'], physical=s['physical'])
 

@@ -95,7 +95,12 @@ def test_resident_64_z_chunk_widths_agree():
     """A 6-load 64^3 FP64 workspace runs the z pass on 17-column kx chunks (its 11-column blocks would need more
     than one round on 256 CUs); a 1-load workspace keeps 11-column chunks. One 6-load launch per stage, the two 3-load
     halves and six 1-load solves certify with the same iteration counts (+-2: the r.z partials regroup) and C^H."""
+    from pgj import _lib
     n = 64
+    grid = _lib.make_grid((n, n, n), (1 / n,) * 3, LAM, MU)
+    assert _lib.lib().pgj_zpass_wide(grid, 6, 8) == 1     # 1152 narrow blocks > 1024 slots of 256 CUs
+    assert _lib.lib().pgj_zpass_wide(grid, 1, 8) == 0
+    assert _lib.lib().pgj_zpass_wide(grid, 6, 4) == 0     # the FP32 pass has no wide instance
     a = torch.tensor(density(n, 1e-6, 4), device='cuda')
     runs = []
     for kw in (dict(load_mode='parallel', batch=6, streams=1), dict(load_mode='parallel', batch=6, streams=2),
@@ -168,3 +173,53 @@ def test_row_march_variants_in_pcg(n, plane):
         assert np.abs(H - design.effective_tensor(a, ref.solutions, C0)).max() / np.abs(H).max() < 1e-10
     finally:
         _lib.check(lib.pgj_hex8_set_shape(r.op.grid, 6, 8, 0, 0, 0, 0))
+
+
+def test_64_wide_and_narrow_preconditioner_match_oracle():
+    """The first direction p0 = P s G s P r of a 6-load set (wide z chunks) and of a 1-load set (narrow) against
+    the oracle's preconditioner on the same residuals."""
+    from pgj import _lib
+    n = 64
+    a = density(n, 1e-6, 4)
+    rng = np.random.default_rng(17)
+    r6 = rng.standard_normal((6, 3 * n ** 3))
+    s = green.scale(a, .5, C0)
+    G6 = green.symbol((n, n, n), hex8.voxel_of((n, n, n)), LAM, MU)
+    ref = green.precond(r6, G6, (n, n, n), s)
+    for kw, loads in ((dict(load_mode='parallel', batch=6, streams=1), 6), (dict(load_mode='serial'), 1)):
+        res = pgj.Resident(n, **kw)
+        assert _lib.lib().pgj_zpass_wide(res.op.grid, res.batch, 8) == (1 if loads == 6 else 0)
+        res.material(torch.tensor(a, device='cuda'))
+        res.select('0.5')
+        res._setup(range(res.batch), 1e-8)
+        res.r.copy_(res.to_soa(torch.tensor(r6[:loads], device='cuda')))
+        res.initialize()
+        p0 = res.to_aos(res.p0).cpu().numpy()
+        err = np.abs(p0 - ref[:loads]).max() / np.abs(ref[:loads]).max()
+        print(f'64^3 preconditioner, {loads} load(s): rel err {err:.2e}', flush=True)
+        assert err < 1e-13
+
+
+def test_64_batched_multi_round_matches_solo():
+    """Two 64^3 designs in one batched set (12 loads: wide z chunks over more than one round of blocks) against
+    each design solved alone in the serial load mode (narrow chunks)."""
+    from pgj.batched import BatchedDesigns
+    n = 64
+    designs = [(density(n, 1e-4, 4, seed=41 + d), alpha) for d, alpha in enumerate((.25, .75))]
+    bd = BatchedDesigns({n: 2}).capture()
+    g = bd.groups[n]
+    for d, (a, alpha) in enumerate(designs):
+        g.material(d, torch.tensor(a, device='cuda'))
+        g.select(d, alpha)
+    infos, _ = bd.solve(1e-8)
+    solo = pgj.Resident(n, load_mode='serial').capture()
+    for d, (a, alpha) in enumerate(designs):
+        solo.solutions.zero_()
+        solo.material(torch.tensor(a, device='cuda'))
+        solo.select(alpha)
+        sinfo, _ = solo.solve(1e-8)
+        Hs = solo.effective_tensor().cpu().numpy()
+        Hb = g.effective_tensor(d).cpu().numpy()
+        assert np.abs(Hb - Hs).max() <= 1e-10 * np.abs(Hs).max()
+        bits = [i['iterations'] for i in infos[n][6 * d:6 * d + 6]]
+        assert max(abs(x - y['iterations']) for x, y in zip(bits, sinfo)) <= 2
