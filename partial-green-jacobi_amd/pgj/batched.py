@@ -139,16 +139,18 @@ class DesignGroup:
         """Row-march shape for this set's launch size and FFT radix plans, timed on its own buffers
         (``pgj.tune``); returns the choices. Call before capturing graphs of this set."""
         from . import tune as tuner
-        self.setup(1e-8)
-        self.r.normal_()
-        self.certify()
-        self.initialize()
-        for k in range(4):
-            self.stage(k, self.p0, self.p1)
-        out = {'row_march': tuner.tune_row_march(self.op.grid, self.L, 8, lambda: self.stage(0, self.p0, self.p1),
-                                                 reps)}
-        if fft:
-            out['fft'] = tuner.tune_fft(self.op.grid, 8, lambda k: self.stage(k, self.p0, self.p1), reps)
+        with tuner.heterogeneous(self.a):
+            self.setup(1e-8)
+            self.r.normal_()
+            self.certify()
+            self.initialize()
+            for k in range(4):
+                self.stage(k, self.p0, self.p1)
+            out = {'row_march': tuner.tune_row_march(self.op.grid, self.L, 8,
+                                                     lambda: self.stage(0, self.p0, self.p1), reps)}
+            if fft:
+                out['fft'] = tuner.tune_fft(self.op.grid, 8, lambda k: self.stage(k, self.p0, self.p1), reps)
+            out['active'] = int(self.active.sum().item())
         torch.cuda.synchronize()
         self.u.zero_()
         self.active.zero_()

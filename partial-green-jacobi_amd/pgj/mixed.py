@@ -156,6 +156,16 @@ class MixedResident:
         from . import tune as tuner
         parts = self._halves.parts
         nl = parts[0][1] if parts else self.L
+        with tuner.heterogeneous(self.a), tuner.heterogeneous(self.a32):
+            out = self._tune(tuner, nl, reps, fft)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.active32.zero_()
+        if self.graphs is not None:
+            self.capture()
+        return out
+
+    def _tune(self, tuner, nl, reps, fft):
         self.setup(1e-8)
         self.u.normal_()
         self.certify()
@@ -172,11 +182,7 @@ class MixedResident:
         out = {'row_march': tuner.tune_row_march(self.op.grid, nl, 4, lambda: stage(0), reps)}
         if fft:
             out['fft'] = tuner.tune_fft(self.op.grid, 4, stage, reps)
-        torch.cuda.synchronize()
-        self.u.zero_()
-        self.active32.zero_()
-        if self.graphs is not None:
-            self.capture()
+        out['active'] = int(self.active32.sum().item())   # loads still active at the end of the timing
         return out
 
     def capture(self):

@@ -190,6 +190,16 @@ class Resident:
         from . import tune as tuner
         parts = self._halves.parts
         nl = parts[0][1] if parts else self.batch
+        with tuner.heterogeneous(self.op._material_field):
+            out = self._tune(tuner, nl, reps, fft)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.solutions.zero_()
+        if self.iter_graph is not None:
+            self.capture()
+        return out
+
+    def _tune(self, tuner, nl, reps, fft):
         self._setup(range(self.batch), 1e-8)
         self.r.normal_()
         self.certify()
@@ -206,11 +216,7 @@ class Resident:
         out = {'row_march': tuner.tune_row_march(self.op.grid, nl, 8, lambda: stage(0), reps)}
         if fft and isinstance(self.n, int):
             out['fft'] = tuner.tune_fft(self.op.grid, 8, stage, reps)
-        torch.cuda.synchronize()
-        self.u.zero_()
-        self.solutions.zero_()
-        if self.iter_graph is not None:
-            self.capture()
+        out['active'] = int(self.active.sum().item())     # loads still active at the end of the timing
         return out
 
     def capture(self):
