@@ -64,6 +64,8 @@ def parse():
     p.add_argument('--profile-stages', action='store_true',
                    help='only launch a 256 MiB device copy (counter calibration) and the four iteration stages '
                         '--probe-iters times each at the bench workload, for rocprofv3 counter runs')
+    p.add_argument('--profile-precision', choices=('fp64', 'fp32'), default='fp64',
+                   help='--profile-stages: the FP64 PCG stages (default) or the FP32 inner stages of the mixed form')
     p.add_argument('--serve-seconds', type=float, default=20.0, help='arrival window of the serve form')
     p.add_argument('--serve-rate', type=float, default=20.0)
     p.add_argument('--serve-clients', type=int, default=64)
@@ -189,6 +191,41 @@ def cpu_baseline(a, alpha, load_iterations, seconds, runs=3):
                       f'{load_iterations:.0f} load-iterations plus 12 certification applies'}
 
 
+def profile_stages_fp32(args):
+    """The four FP32 inner-iteration stages of the mixed-precision form (one design, six loads) launched
+    ``--probe-iters`` times each with every load active: the counter-run workload of the FP32 kernels."""
+    from pgj import _lib
+    from pgj.density import filtered_noise, simp
+    from pgj.mixed import MixedResident
+
+    n = args.n
+    m = MixedResident(n)
+    m.material(0, torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda'))
+    m.select(0, args.alpha)
+    m.setup(1e-8)
+    m.certify()
+    m.begin()
+
+    def stage(k):
+        _lib.call('pgj_inner_stage', m.op.grid, m.L, m.G, 0, m.L, _lib.ptr(m.plan32), _lib.ptr(m.a32),
+                  _lib.ptr(m.scale32), _lib.ptr(m.scale_hat32), _lib.ptr(m.d), _lib.ptr(m.r32), _lib.ptr(m.p0),
+                  _lib.ptr(m.p1), _lib.ptr(m.q), _lib.ptr(m.state32), _lib.ptr(m.ws32), k, _lib.stream())
+
+    for k in range(4):
+        stage(k)
+    out = {}
+    for k, name in enumerate(['hex8_rowmarch', 'fxy', 'fz', 'fxyi']):
+        stage(k)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.probe_iters):
+            stage(k)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / args.probe_iters
+    print(json.dumps({'stages_ms_fp32': out, 'active': m.active32.cpu().numpy().tolist(), 'loads': m.L}), flush=True)
+
+
 def profile_stages(args):
     """Counter-run workload: one 256 MiB copy (read 256 MiB, write 256 MiB) and the four stages."""
     import pgj
@@ -198,6 +235,8 @@ def profile_stages(args):
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
+    if args.profile_precision == 'fp32':
+        return profile_stages_fp32(args)
     n = args.n
     res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=args.batch, streams=args.streams).capture()
     res.material(torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda'))
