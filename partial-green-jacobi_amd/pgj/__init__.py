@@ -9,8 +9,8 @@ from .certify import CertificationError
 from .driver import compute_effective_stiffness_batched
 from .operator import Operator, build, diagonal, scale, set_alpha
 from .pcg import certified_solve, solve_pcg_batched
-from .resident import PaperResident, Resident
+from .resident import PaperResident, Resident, SelectorResident
 
-__all__ = ['Candidates', 'CertificationError', 'Operator', 'PaperBackend', 'PaperResident', 'Resident', 'build',
-           'certified_solve', 'compute_effective_stiffness_batched', 'diagonal', 'pilot', 'scale', 'set_alpha',
-           'solve_pcg_batched']
+__all__ = ['Candidates', 'CertificationError', 'Operator', 'PaperBackend', 'PaperResident', 'Resident',
+           'SelectorResident', 'build', 'certified_solve', 'compute_effective_stiffness_batched', 'diagonal', 'pilot',
+           'scale', 'set_alpha', 'solve_pcg_batched']

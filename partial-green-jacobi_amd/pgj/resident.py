@@ -377,3 +377,15 @@ class PaperResident(Resident):
     def energy(self, iteration=0):
         """(objective, local (n, n, n), diagonal (6,)) (``reproduction.py:153-164``)."""
         return self.objective_energy(self.objective_kind, iteration)
+
+    def select(self, method, emin=None, rho=None):
+        """``Resident.select`` where the 'eta' table reads the outer loop's Emin and the 'gray' selector its
+        physical density, as the reference's selector resident does
+        (``exp/asg_density_selector_20261105/run.py:46-77``)."""
+        return super().select(method, self.outer.emin if emin is None else emin,
+                              self.outer.ph if rho is None else rho)
+
+
+# The density-selector resident of exp/asg_density_selector_20261105/run.py:46 (PaperResident with the
+# selectors; here every Resident selects on the device).
+SelectorResident = PaperResident

@@ -134,3 +134,51 @@ def test_reference_constructor_errors():
     bad.C0 = np.arange(36.0).reshape(6, 6)
     with pytest.raises(ValueError):
         pgj.Resident(bad, 8)
+
+
+def test_replay_state_call_sequence(tmp_path):
+    """examples/replay_state.py:38-59 through pgj: a Study-II state file with the archive's keys (a, rho, u0),
+    SelectorResident(backend, n, objective, .2, rmin=3.2, emin=eta, load_mode), material + outer.ph from the
+    file, and for each method: select, warm solve from u0, reset to u0, timed select + solve, C^H. Every method
+    certifies, selects the oracle's alpha and gives the same C^H; C^H matches the oracle's solve."""
+    from oracle import select as osel
+    from pgj import checkpoint
+    n, eta = 16, 1e-6
+    outer = pgj.outer.PaperOuter(n, .2, rmin=3.2, emin=eta).capture()
+    outer.set_initial(torch.tensor(pgj.density.filtered_noise(n, 5, 2), device='cuda'))
+    outer.evaluate()
+    a, rho = outer.a.cpu().numpy(), outer.ph.cpu().numpy()
+    C0 = hex8.isotropic(15 / 26, 5 / 13)
+    warm = pcg.ResidentPCG(a, C0, np.eye(6), batch=6, s=green.scale(a, .5, C0))
+    warm.solve(rtol=1e-4)
+    path = tmp_path / 'state.npz'
+    checkpoint.save_state(path, a=a, rho=rho, u0=warm.solutions)
+    state = np.load(path, allow_pickle=False)
+    for mode in ('serial', 'parallel'):
+        print(f'replay_state call sequence, {mode}', flush=True)
+        r = pgj.SelectorResident(pgj.PaperBackend(), n, 'shear', .2, rmin=3.2, emin=eta, load_mode=mode)
+        r.capture()
+        r.material(torch.as_tensor(state['a'], device='cuda'))
+        r.outer.ph.copy_(torch.as_tensor(state['rho'], device='cuda'))
+        u0 = torch.as_tensor(state['u0'], device='cuda')
+        Cs = {}
+        for method in ('gj', 'eta', 'compact8', 'gray', '0.5'):
+            r.select(method)
+            r.solutions.copy_(u0)
+            r.solve()
+            r.solutions.copy_(u0)
+            r.select(method)
+            loads, checks = r.solve()
+            assert len(loads) == 6 and all(i['converged'] for i in loads)
+            Cs[method] = r.effective_tensor().cpu().numpy()
+            want = {'gj': 1.0, 'eta': osel.eta_alpha(eta), 'compact8': osel.compact8_alpha(a),
+                    'gray': osel.gray_alpha(a, rho), '0.5': 0.5}[method]
+            assert abs(float(r.alpha) - want) <= 1e-6, (method, float(r.alpha), want)
+        H = Cs['0.5']
+        for method, C in Cs.items():
+            assert np.abs(C - H).max() <= 1e-9 * np.abs(H).max(), method
+        ref = pcg.ResidentPCG(a, C0, np.eye(6), batch=3, s=green.scale(a, .5, C0))
+        ref.solutions[:] = warm.solutions
+        ref.solve(rtol=1e-8)
+        Href = design.effective_tensor(a, ref.solutions, C0)
+        assert np.abs(H - Href).max() <= 1e-10 * np.abs(Href).max()
