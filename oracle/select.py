@@ -54,21 +54,27 @@ def eta_alpha(emin, anchors=ETA_ANCHORS):
 
 
 def law_features(a):
-    """(area, jump, contrast, complexity) of the field in FP32; axis 0 is the field's first axis."""
-    ell = np.log(np.asarray(a, np.float32))
+    """(area, jump, contrast, complexity) of the field, computed and reduced in FP32 as the reference
+    (``solver/experiments/global_alpha_compact_law/law.py:10-19``, float32 tensors and float32 means), then
+    widened to FP64 (``resident.py:164``); axis 0 is the field's first axis."""
+    f32 = np.float32
+    ell = np.log(np.asarray(a, f32))
     lo, hi = ell.min(), ell.max()
-    span = np.float32(hi - lo)
-    q = (ell - lo) / np.maximum(span, np.float32(1e-12))
+    span = f32(hi - lo)
+    q = (ell - lo) / np.maximum(span, f32(1e-12))
     n = q.shape[0]
     s1, s2, active = [], [], []
     for axis in range(3):
         d = np.abs(np.roll(q, -1, axis) - q)
-        s1.append(d.astype(np.float64).mean())
-        s2.append(np.square(d).astype(np.float64).mean())
-        active.append((d > np.float32(1e-6)).mean())
-    total1, total2, area = sum(s1), sum(s2), sum(active) / 3
-    jump = total2 / max(total1, 1e-20)
-    return np.array([area, jump, float(span) / math.log(10000.), area * n])
+        s1.append(d.mean(dtype=f32))
+        s2.append(np.square(d).mean(dtype=f32))
+        active.append((d > f32(1e-6)).astype(f32).mean(dtype=f32))
+    total1 = f32(f32(s1[0] + s1[1]) + s1[2])
+    total2 = f32(f32(s2[0] + s2[1]) + s2[2])
+    area = f32(f32(f32(active[0] + active[1]) + active[2]) / f32(3))
+    jump = f32(total2 / max(total1, f32(1e-20)))
+    out = np.array([area, jump, f32(span / f32(math.log(10000.))), f32(area * f32(n))], dtype=f32)
+    return out.astype(np.float64)
 
 
 def compact8_alpha(a, quantize=False):

@@ -174,18 +174,20 @@ __global__ void alpha_finish_kernel(int method, AlphaParams prm, const double* s
   }
   const double s = fmin(fmax(log10(hi / lo), 0.0), 9.0);
   const double mr = r1 / double(N), mr2 = r2 / double(N);
-  // law features: per-axis means, accumulated over the axes in order z, y, x
+  // law features in the reference's float32 flow (law.py:10-19): per-axis float32 means (each the float
+  // rounding of an FP64 sum), float32 totals over the axes in order z, y, x, float32 ratios; widened after
+  auto f32 = [](double v) { return double(float(v)); };
   double total1 = 0.0, total2 = 0.0, act = 0.0;
   for (int ax = 0; ax < 3; ++ax) {
-    total1 += sums[3 * ax] / double(N);
-    total2 += sums[3 * ax + 1] / double(N);
-    act += sums[3 * ax + 2] / double(N);
+    total1 = f32(total1 + f32(sums[3 * ax] / double(N)));
+    total2 = f32(total2 + f32(sums[3 * ax + 1] / double(N)));
+    act = f32(act + f32(sums[3 * ax + 2] / double(N)));
   }
-  const double area = act / 3.0;
-  const double jump = total2 / fmax(total1, 1e-20);
-  const double lspan = double(float(lhi) - float(llo));
-  const double lcontrast = lspan / log(10000.0);
-  const double complexity = area * double(n_axis0);
+  const double area = f32(act / 3.0);
+  const double jump = f32(total2 / fmax(total1, 1e-20));
+  const float lspan = float(lhi) - float(llo);
+  const double lcontrast = double(lspan / float(log(10000.0)));
+  const double complexity = f32(area * double(n_axis0));
   double al = 0.0;
   switch (method) {
     case 0: al = prm.v[0]; break;                 // fixed (also eta: anchor chosen on the host)
