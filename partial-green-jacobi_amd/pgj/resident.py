@@ -196,7 +196,7 @@ class Resident:
         self.u.zero_()
         self.solutions.zero_()
         if self.iter_graph is not None:
-            self.capture()
+            self._capture_solver()          # the solver's graphs only: the outer loop's design stays as it is
         return out
 
     def _tune(self, tuner, nl, reps, fft):
@@ -220,9 +220,13 @@ class Resident:
         return out
 
     def capture(self):
-        """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``), and the outer loop's."""
+        """Capture init / chunk-of-iterations / certify graphs (``resident.py:138-150``), and the outer loop's
+        (which, as the reference's, warms on a test density: set the outer's initial design after capture)."""
         if self.outer is not None:
             self.outer.capture()
+        return self._capture_solver()
+
+    def _capture_solver(self):
         self._setup(range(self.batch), 1e-8)
         self.r.normal_()
         self.certify()

@@ -248,8 +248,9 @@ class Server:
         return self._finalize(n, slot, t_end, diag)
 
     def _fail(self, n, slot, t_end):
-        """A step that failed certification: record it, keep the client's design and warm start, free the slot
-        if the client has no step left, and go on (as ``SequentialServer`` does)."""
+        """A step that failed certification: record it, keep the client's density (no OC update), free the slot
+        if the client has no step left, and go on (as ``SequentialServer`` does). The slot keeps the failed
+        solve's last iterate as the next warm start (the sequential baseline keeps the previous certified one)."""
         req = self.solving[n][slot]
         c = req.client
         req.end = t_end
