@@ -55,12 +55,13 @@ def parse():
                    help='skip the served 64^3 batched mixed-precision stream against the plain path')
     p.add_argument('--served-designs', type=int, default=16)
     p.add_argument('--served-steps', type=int, default=3)
-    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid', 'configs', 'served64'),
+    p.add_argument('--form', choices=('plain', 'batched', 'serve', 'mixed', 'hybrid', 'configs', 'served64', 'mfma'),
                    default='plain',
                    help='plain: the FP64 drop-in design step (default); batched: 48 designs of uneven grids in one '
                         'captured graph vs the same designs solved one at a time; serve: Poisson-arrival '
                         'design-step requests through the continuous-batching server; mixed: FP32-inner / '
-                        'FP64-refinement solves at 64^3 and 128^3 against the plain FP64 solve')
+                        'FP64-refinement solves at 64^3 and 128^3 against the plain FP64 solve; mfma: the FP32 '
+                        'HEX8 apply as a dense Ke contraction on the matrix cores against the modal row-march')
     p.add_argument('--profile-stages', action='store_true',
                    help='only launch a 256 MiB device copy (counter calibration) and the four iteration stages '
                         '--probe-iters times each at the bench workload, for rocprofv3 counter runs')
@@ -768,6 +769,81 @@ def serve_form(args):
     print(json.dumps(out), flush=True)
 
 
+FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix rate
+
+
+def mfma_form(args):
+    """The FP32 HEX8 apply at the bench workload (n^3, six loads): the modal row-march (the inner operator of the
+    mixed form) against the dense 24x24 Ke contraction on v_mfma_f32_16x16x4_f32, AoS and SoA, each launch timed
+    with HIP events on the launch stream."""
+    import pgj
+    from pgj import _lib
+    from pgj.density import filtered_noise, simp
+
+    n, L = args.n, 6
+    N = n ** 3
+    a = torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda')
+    op = pgj.Operator(a, dtype=torch.float32)
+    u = torch.randn(L, 3 * N, dtype=torch.float32, device='cuda', generator=torch.Generator('cuda').manual_seed(7))
+    soa = u.reshape(L, N, 3).transpose(1, 2).contiguous().reshape(L, 3 * N)
+    outs = {k: torch.empty_like(u) for k in ('rowmarch', 'mfma_aos', 'mfma_soa')}
+    calls = {
+        'rowmarch': lambda: _lib.call('pgj_hex8_apply32', op.grid, L, _lib.ptr(op._material32), _lib.ptr(u), None,
+                                      _lib.ptr(outs['rowmarch']), _lib.stream()),
+        'mfma_aos': lambda: _lib.call('pgj_hex8_apply32_mfma', op.grid, L, _lib.ptr(op._material32), _lib.ptr(u),
+                                      None, _lib.ptr(outs['mfma_aos']), 0, _lib.stream()),
+        'mfma_soa': lambda: _lib.call('pgj_hex8_apply32_mfma', op.grid, L, _lib.ptr(op._material32),
+                                      _lib.ptr(soa), None, _lib.ptr(outs['mfma_soa']), 1, _lib.stream()),
+    }
+    if args.profile_stages:
+        for k in calls:
+            for _ in range(1 + args.probe_iters):
+                calls[k]()
+        torch.cuda.synchronize()
+        print(json.dumps({'launches': 1 + args.probe_iters, 'variants': list(calls)}), flush=True)
+        return
+    us = {}
+    for k, call in calls.items():
+        call()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(20)]
+        for e0, e1 in ev:
+            e0.record()
+            call()
+            e1.record()
+        torch.cuda.synchronize()
+        us[k] = 1e3 * float(np.mean([e0.elapsed_time(e1) for e0, e1 in ev]))
+    ref = outs['rowmarch'].double()
+    scale = float(ref.abs().max())
+    diff_aos = float((outs['mfma_aos'].double() - ref).abs().max()) / scale
+    back = outs['mfma_soa'].reshape(L, 3, N).transpose(1, 2).reshape(L, 3 * N).double()
+    diff_soa = float((back - ref).abs().max()) / scale
+    # matrix-core work per launch: 12 instructions of 16 x 16 x 4 per 16 elements of an element row, over the
+    # owned rows plus one halo row per 4 and the owned layers plus one halo layer per z march
+    zc = next(c for c in (8, 4, 2, 1) if n % c == 0)
+    while zc > 1 and (n // 4) * (n // zc) * L < 768:
+        zc //= 2
+    halo = (5 / 4) * ((zc + 1) / zc)
+    issued = 2 * 16 * 16 * 4 * 12 / 16 * N * L * halo
+    useful = 2 * 576 * N * L
+    bytes_ = (24 * L + 4) * N
+    t = us['mfma_soa']
+    out = {
+        'metric': f'FP32 HEX8 apply time at {n}^3 x {L} loads (modal row-march vs dense Ke on matrix cores)',
+        'value': us['rowmarch'] / t, 'unit': 'speedup of the matrix-core apply (SoA) over the row-march',
+        'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32', 'data': 'synthetic (bench density, seeded u)',
+        'config': {'workload': f'{n}^3, Emin {args.emin}, radius {args.radius} filtered noise, {L} loads',
+                   'mfma_shape': 'v_mfma_f32_16x16x4_f32, 16 elements x 24 DOFs per tile, 4 node rows x all x '
+                                 f'per block, {zc} layers per z march'},
+        'apply_us': us,
+        'max_rel_diff_vs_rowmarch': {'mfma_aos': diff_aos, 'mfma_soa': diff_soa},
+        'mfma': {'issued_TFLOPs': issued / t / 1e6, 'useful_TFLOPs': useful / t / 1e6,
+                 'peak_TFLOPs': FP32_MFMA_PEAK_TFLOPS, 'issued_frac': issued / t / 1e6 / FP32_MFMA_PEAK_TFLOPS,
+                 'issue_floor_us': issued / FP32_MFMA_PEAK_TFLOPS / 1e6},
+        'hbm_GBs': {k: bytes_ / v / 1e3 for k, v in us.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -789,6 +865,9 @@ def main():
     if args.form == 'configs':
         torch.cuda.set_device(0)
         return configs_form(args)
+    if args.form == 'mfma':
+        torch.cuda.set_device(0)
+        return mfma_form(args)
     if args.form == 'served64':
         torch.cuda.set_device(0)
         print(json.dumps(served_64(args)), flush=True)

@@ -62,6 +62,15 @@ int pgj_hex8_apply_reference(const pgj_grid* g, int L, const double* a, const do
  * Needs a grid the row-march supports. */
 int pgj_hex8_apply32(const pgj_grid* g, int L, const float* a, const float* u, const uint8_t* active, float* out,
                      void* stream);
+/* The same FP32 operator as a dense 24x24 Ke contraction on the matrix cores
+ * (v_mfma_f32_16x16x4_f32 over 16 elements x 24 DOFs per tile): the element
+ * product of `apply_scalar` (exp/asg_sctop_reproduction_20261105/scalar_fi.cu:14-36)
+ * with the dense Ke of exp/asg_candidates_20261103/candidates.py:30-54 in place
+ * of the matrix-free modal form. soa != 0: vectors are (L, 3, N) component-major
+ * (the resident solver's layout), else (L, N, 3). Needs nx = 64 or 128 and ny
+ * divisible by 4. */
+int pgj_hex8_apply32_mfma(const pgj_grid* g, int L, const float* a, const float* u, const uint8_t* active,
+                          float* out, int soa, void* stream);
 
 /* f = -sum_e a_e Ke chi(macro), shape (1, 3N); macro is 6 host doubles in
  * Voigt order with engineering shear. Replaces `rhs`

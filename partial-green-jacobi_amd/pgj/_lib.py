@@ -31,6 +31,7 @@ SIGNATURES = {
     'pgj_last_error': (ctypes.c_char_p, []),
     'pgj_hex8_apply': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_hex8_apply32': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hex8_apply32_mfma': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     'pgj_hex8_apply_reference': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     'pgj_hex8_rhs': (c_int, [GP, c_void_p, c_dp, c_void_p, c_void_p]),
     'pgj_hex8_energy': (c_int, [GP, c_int, c_void_p, c_dp, c_void_p, c_void_p]),

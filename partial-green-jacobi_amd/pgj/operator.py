@@ -98,6 +98,20 @@ class Operator:
         self._apply_call(u.shape[0], u, active, out)
         return out
 
+    def apply_batched_mfma(self, u, active=None):
+        """``apply_batched`` of the FP32 operator with the element product as a dense 24x24 Ke contraction on
+        the matrix cores (``pgj_hex8_apply32_mfma``) instead of the modal row-march."""
+        if self.dtype != torch.float32:
+            raise ValueError('the matrix-core apply is the FP32 operator')
+        if u.dim() != 2 or u.shape[1] != self.mesh.n_dofs or u.dtype != self.dtype or not u.is_contiguous():
+            raise ValueError('Expected contiguous displacements (L,3N) of the operator dtype')
+        out = torch.empty_like(u)
+        if active is not None:
+            active = active.to(torch.uint8).contiguous()
+        _lib.call('pgj_hex8_apply32_mfma', self.grid, u.shape[0], _lib.ptr(self._material32), _lib.ptr(u),
+                  _lib.ptr(active), _lib.ptr(out), 0, _lib.stream())
+        return out
+
     def apply_reference(self, u):
         """Same operator by the independent thread-per-node kernel (certification operator)."""
         out = torch.empty_like(u)
