@@ -131,10 +131,14 @@ int pgj_diagonal_scale(const pgj_grid* g, const double* a, double alpha, double*
 int pgj_hex8_set_shape(const pgj_grid* g, int L, int es, int R, int W, int ZC, int plane);
 /* Radix plan of one FFT pass of the resident PCG iteration on the cube g in
  * precision es (8: FP64, 4: FP32): pass 0 forward x/y, 1 z with the Green
- * multiply, 2 inverse y/x; plan 0 takes radix-4 stages first (64 = 4 x 4 x 4,
- * the default), plan 1 radix-8 stages first (64 = 8 x 8). The choice holds for
- * every later launch of that pass on any n^3 grid of that precision (a graph
- * keeps the kernels of its capture). get_plan writes the three passes' plans.
+ * multiply, 2 inverse y/x; plan 0 takes radix-4 stages first (64 = 4 x 4 x 4),
+ * plan 1 radix-8 stages first (64 = 8 x 8); for the z pass at 64^3, plan 2
+ * keeps the transforms in registers (two radix-8 passes with one LDS transpose
+ * each, LDS otherwise only for the 3x3 Green multiply). plan -1 restores the
+ * default: plan 0, except plan 2 for the FP32 z pass at 64^3. The
+ * choice holds for every later launch of that pass on any n^3 grid of that
+ * precision (a graph keeps the kernels of its capture; a solver reads the plan
+ * when it sets up its launches). get_plan writes the three passes' plans.
  * Counterpart of the reference's variant sweep
  * (solver/src/hex8_modal/gpu_study.py:174-222) for the FFT radix schedule. */
 int pgj_fft_set_plan(const pgj_grid* g, int es, int pass, int plan);

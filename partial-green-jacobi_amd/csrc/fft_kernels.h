@@ -10,6 +10,8 @@ namespace pgj {
 enum { kFxyScale = 0, kFxyPre = 1, kFxyPCG = 2 };
 enum { kFzScale = 0, kFzPre = 1, kFzInit = 2, kFzPCG = 3 };
 enum { kFxyiPre = 0, kFxyiPCG = 1 };
+// FFTArgs::zwide value of the register-resident z pass of the 64^3 cube (fz64_reg).
+constexpr int kZReg = 2;
 
 // Arguments of the FFT/Green kernels; T is the vector precision (FP64 plain
 // path, FP32 inner solver of the mixed-precision form). Partial sums and the
@@ -40,8 +42,9 @@ struct FFTArgsT {
   StateView st;
   int soa;               // vectors (and s) are SoA (L, 3, N) instead of AoS (L, N, 3)
   int group;             // loads per design (s is (designs, 3N), s_hat (designs, 3, nf)); 0: shared
-  int zwide;             // compile-time z pass of a cube on its wide kx chunks (zpass_wide); the same for
-                         // every launch that writes or reads one workspace's z partials
+  int zwide;             // z pass variant of a cube (zpass_variant): 0 default kx chunks, 1 wide chunks,
+                         // kZReg register-resident; the same for every launch that writes or reads one
+                         // workspace's z partials
 };
 using FFTArgs = FFTArgsT<double>;
 
@@ -61,6 +64,9 @@ bool static_fft(const pgj_grid& g);
 // takes the wide kx chunks: where the default chunks need more than one round of blocks on the device
 // (64^3 FP64 from 6 loads on 256 CUs).
 bool zpass_wide(const pgj_grid& g, int L, int es);
+// The z pass variant of the compile-time kernels for such a solve: kZReg where the 64^3 z pass runs on its
+// register-resident plan (pgj_fft_set_plan pass 1, plan 2), else the wide chunks where zpass_wide holds, else 0.
+int zpass_variant(const pgj_grid& g, int L, int es);
 template <typename T> int launch_fxy_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
 template <typename T> int launch_fz_static(FFTArgsT<T>& A, int L, int mode, hipStream_t s);
 template <typename T> int launch_fxyi_static(FFTArgsT<T>& A, int L, hipStream_t s, int mode = kFxyiPCG);

@@ -82,10 +82,17 @@ struct ZPass {
 template <int N, typename R>
 constexpr int kWideKC = N == 64 && sizeof(R) == 8 ? 17 : 0;
 
-// Chunks per ky row of the z pass, which lays out its per-block partials.
+// Register-resident z pass of the 64^3 cube (fz64_reg below): blocks per load, which lay out its partials:
+// 4 chunks of 8 kx columns per ky row, then 8 blocks for the Nyquist column kx = 32 (8 ky rows each).
+constexpr int kZRegBlocks = 4 * 64 + 8;
+
+// Per-load partial slots written by the z pass variant (FFTArgs::zwide: 0 default chunks, 1 wide chunks,
+// kZReg the register-resident pass); fxy and fxyi reduce exactly these.
 template <int N, typename R>
-__host__ __device__ constexpr int z_chunks(bool wide) {
-  return kWideKC<N, R> > 0 && wide ? ZPass<N, R, (kWideKC<N, R> > 0 ? kWideKC<N, R> : 1)>::NKB : ZPass<N, R>::NKB;
+__host__ __device__ constexpr int z_slots(int variant) {
+  if (N == 64 && variant == kZReg) return kZRegBlocks;
+  return (kWideKC<N, R> > 0 && variant == 1 ? ZPass<N, R, (kWideKC<N, R> > 0 ? kWideKC<N, R> : 1)>::NKB
+                                            : ZPass<N, R>::NKB) * N;
 }
 
 // Global -> LDS copy of COUNT values in two phases, so that independent work
@@ -167,10 +174,9 @@ void fxy_static(FFTArgsT<R> A) {
     // PCG: the step length is reduced while the r and q loads are in flight
     if (MODE == kFxyPCG && i0 == 0) {
       double pq, rz;
-      const int NKB = z_chunks<N, R>(A.zwide);
       // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
       wave_reduce_partials_2(A.ppq + int64_t(l) * A.ppq_stride, A.ppq_count, 1, A.pg + int64_t(l) * A.f.nkb * N * 4,
-                             NKB * N, 4, pq, rz);
+                             z_slots<N, R>(A.zwide), 4, pq, rz);
       const bool valid = isfinite(pq) && pq > 0.0 && isfinite(rz) && rz > 0.0;
       alpha = valid ? rz / pq : 0.0;
       if (threadIdx.x == 0 && z == 0 && c == 0) {
@@ -431,6 +437,167 @@ void fz_static(FFTArgsT<R> A) {
   }
 }
 
+// ------------------------------------------------------- z + Green, in registers (n = 64)
+// The z pass of the 64^3 cube with its transforms in registers. A block is three waves, wave c holding
+// component c of 8 kx columns of one ky row (the Nyquist blocks: column kx = 32 of 8 ky rows); lane
+// (kcol = lane % 8, j = lane / 8) holds the points kz = j + 8 r (r < 8) of its column. The 64-point
+// transform is two radix-8 Stockham passes: pass 1 is a DFT over the lane's own points, an 8 x 8 transpose
+// among the 8 lanes of a column goes through LDS, and pass 2 (twiddles W^(j r)) leaves the lane with the
+// frequencies kz = j + 8 r: the same points the first inverse pass reads. LDS carries the two transposes of
+// each transform and the exchange of the three components for the 3x3 Green multiply; the default pass
+// stages every point through LDS for each of its radix stages. Semantics are those of fz_static.
+template <int MODE, typename R>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void fz64_reg(FFTArgsT<R> A) {
+  // LDS rows of one column, padded so that the 16 lanes (8 columns x 2 j) of a transposed or exchange read
+  // of 16-byte values fall on distinct banks
+  constexpr int N = 64, NKX = N / 2 + 1, RS = N + 2;
+  constexpr int64_t NF = int64_t(N) * N * NKX;
+  using X = cplx<R>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  X* buf = reinterpret_cast<X*>(smem);                       // [component][kcol][RS]
+  X* twl = buf + 3 * 8 * RS;                                  // W_64^k, k < 64
+  double* red = reinterpret_cast<double*>(twl + N);           // block sums: 4 values x 3 waves
+  const int c = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  const int lane = int(threadIdx.x) & 63, kcol = lane & 7, j = lane >> 3;
+  const int b = int(blockIdx.x), l = int(blockIdx.z);
+  const int ky = b < 256 ? b % 64 : 8 * (b - 256) + kcol;
+  const int kx = b < 256 ? 8 * (b / 64) + kcol : N / 2;
+  const bool lead = threadIdx.x == 0 && b == 0;
+  const int64_t NN = int64_t(N) * N * N;
+  if (MODE == kFzPCG && !A.st.active[l]) {
+    if (lead) {
+      A.st.active_next[l] = 0;
+      A.st.it_next[l] = A.st.it[l];
+      A.st.bad_next[l] = A.st.bad[l];
+      A.st.stopped[l] = 0;
+    }
+    return;
+  }
+  X* S = A.spec + int64_t(l) * 3 * NF;
+  auto fidx = [&](int kz) { return (int64_t(kz) * N + ky) * NKX + kx; };
+  // the column's points and the twiddle table are in flight through the convergence test
+  X v[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = S[int64_t(c) * NF + fidx(j + 8 * r)];
+  const X twv = A.twz[lane];
+  // s_hat_c at the lane's frequencies, in flight through the convergence test and the forward transform
+  const X* __restrict__ shat_d = A.shat ? A.shat + int64_t(design_of(l, A.group)) * 3 * NF : nullptr;
+  X sh[8];
+  if (MODE != kFzScale) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sh[r] = shat_d ? shat_d[int64_t(c) * NF + fidx(j + 8 * r)] : c2(R(0), R(0));
+  }
+  double mean_c = 0.0;
+  if (MODE == kFzInit || MODE == kFzPCG) {
+    double res2 = 0.0, vm = 0.0;
+    {
+      double t = 0.0, m = 0.0;
+      for (int i = lane; i < N; i += 64) {
+        const double* p = A.pr + (int64_t(l) * N + i) * 6;
+        t += (p[0] + p[2]) + p[4];
+        m += p[2 * c + 1];
+      }
+      res2 = wave_sum(t);
+      vm = wave_sum(m);
+    }
+    if (A.shat) mean_c = vm / double(NN);
+    bool act;
+    if (MODE == kFzInit) {
+      act = res2 > A.st.target2[l] && A.st.it[l] < A.st.cap[l] && !A.st.bad[l];
+      if (lead) {
+        A.st.active_next[l] = act;
+        A.st.it_next[l] = A.st.it[l];
+        A.st.bad_next[l] = A.st.bad[l];
+        A.st.first[l] = 1;
+        A.st.stopped[l] = 0;
+      }
+    } else {
+      const bool active = A.st.active[l] != 0;
+      const bool valid = A.st.valid[l] != 0;
+      const bool finite = isfinite(res2);
+      const int64_t it = A.st.it[l] + (active ? 1 : 0);
+      act = active && valid && finite && res2 > A.st.target2[l] && it < A.st.cap[l];
+      if (lead) {
+        A.st.active_next[l] = act;
+        A.st.it_next[l] = it;
+        A.st.bad_next[l] = A.st.bad[l] || (active && !valid) || (active && !finite);
+        A.st.stopped[l] = active && !act;
+      }
+    }
+    if (!act) return;
+  } else if (MODE != kFzScale && A.shat) {
+    mean_c = wave_reduce_partials(A.pr + (int64_t(l) * N * 3 + c) * 2 + 1, N, 6) / double(NN);
+  }
+  if (c == 0) twl[lane] = twv;
+  X* row = buf + (c * 8 + kcol) * RS;
+  const X* none = nullptr;
+  // forward: pass 1 over x[j + 8 r] -> y[8 j + r], transpose, pass 2 (twiddles W^(j r)) -> Y[j + 8 r]
+  dft<8, false>(v, none, N, 1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) row[8 * j + r] = v[r];
+  // this wave's row of the symbol (G_c0, G_c1, G_c2 of [G00, G11, G22, G01, G02, G12]), in flight through
+  // the transpose, the second pass and the component exchange
+  R g[8][3];
+  if (MODE != kFzScale) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int64_t f = fidx(j + 8 * r);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) g[r][d] = A.sym[(d == c ? c : c + d + 2) * NF + f];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = r == 0 ? row[j] : cmul(row[j + 8 * r], twl[(j * r) & (N - 1)]);
+  dft<8, false>(v, none, N, 1);
+  if (MODE == kFzScale) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) S[int64_t(c) * NF + fidx(j + 8 * r)] = v[r];
+    return;
+  }
+  // projection mean correction, then the three components meet in LDS for the Green multiply
+  if (shat_d) {
+    const R mc = R(mean_c);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = c2(fma(-mc, sh[r].x, v[r].x), fma(-mc, sh[r].y, v[r].y));
+  }
+  // a lane writes back exactly the slots it read after the transpose: no other lane reads them again
+#pragma unroll
+  for (int r = 0; r < 8; ++r) row[j + 8 * r] = v[r];
+  __syncthreads();
+  const double wgt = (kx == 0 || kx == N / 2) ? 1.0 : 2.0;
+  double rz = 0.0, sw = 0.0;
+  X w[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int kz = j + 8 * r;
+    const X v0 = buf[(0 * 8 + kcol) * RS + kz], v1 = buf[(1 * 8 + kcol) * RS + kz], v2 = buf[(2 * 8 + kcol) * RS + kz];
+    w[r] = c2(g[r][0] * v0.x + g[r][1] * v1.x + g[r][2] * v2.x, g[r][0] * v0.y + g[r][1] * v1.y + g[r][2] * v2.y);
+    rz = fma(wgt, double(v[r].x) * double(w[r].x) + double(v[r].y) * double(w[r].y), rz);
+    if (shat_d) sw = fma(wgt, double(sh[r].x) * double(w[r].x) + double(sh[r].y) * double(w[r].y), sw);
+  }
+  __syncthreads();  // the exchange rows are read before the inverse transposes overwrite them
+  // inverse: pass 1 over W[j + 8 r] -> 8 j + r, transpose, pass 2 with conjugate twiddles -> z = j + 8 r
+  dft<8, true>(w, none, N, 1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) row[8 * j + r] = w[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) w[r] = r == 0 ? row[j] : cmul(row[j + 8 * r], conjd(twl[(j * r) & (N - 1)]));
+  dft<8, true>(w, none, N, 1);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) S[int64_t(c) * NF + fidx(j + 8 * r)] = w[r];
+  double t[4] = {rz, c == 0 ? sw : 0.0, c == 1 ? sw : 0.0, c == 2 ? sw : 0.0};
+  block_sums<4>(t, red);
+  if (threadIdx.x == 0) {
+    double* pg = A.pg + (int64_t(l) * A.f.nkb * N + b) * 4;
+    pg[0] = t[0];
+    pg[1] = t[1];
+    pg[2] = t[2];
+    pg[3] = t[3];
+  }
+}
+
 // --------------------------------------------------------------- inverse xy
 // MODE kFxyiPre: the preconditioner apply alone (z = P(s w), no PCG state).
 template <int N, int MODE, typename R, int PLAN = 0>
@@ -471,7 +638,7 @@ void fxyi_static(FFTArgsT<R> A) {
   StageIn<N, T, X> twl;
   in.load(A.spec + ((int64_t(l) * 3 + c) * N + z) * int64_t(N) * C::NKX, [](int e) { return e; });
   twl.load(A.twy, [](int e) { return e; });
-  const int NKB = z_chunks<N, R>(A.zwide);
+  const int NSLOT = z_slots<N, R>(A.zwide);
   // per-load partials are laid out with the geometry's chunk count (slices of a set are offset by it)
   const double* pgl = A.pg + int64_t(l) * A.f.nkb * N * 4;
   // the output is the next direction p = z + beta p (z alone right after init or without a direction input)
@@ -481,7 +648,7 @@ void fxyi_static(FFTArgsT<R> A) {
     // one pass: r.z (offset 0) and this component's s.w (offset 1 + c) partials
     const int off[2] = {0, 1 + c};
     double sums[2];
-    wave_reduce_partials_n<2>(pgl, off, NKB * N, 4, sums);
+    wave_reduce_partials_n<2>(pgl, off, NSLOT, 4, sums);
     if (A.shat) mean = sums[1] / double(NN);
     if (update) {
       const double rz = A.st.rz[l];
@@ -640,10 +807,29 @@ int fz_static_kc(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
   return PGJ_OK;
 }
 
+template <typename R>
+int fz64_reg_launch(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
+  constexpr size_t lds = (size_t(3) * 8 * 66 + 64) * sizeof(cplx<R>) + 4 * 8 * sizeof(double);
+  PGJ_REQUIRE(kZRegBlocks <= A.f.nkb * 64, "z pass blocks (%d) exceed the partial layout (%d)", kZRegBlocks,
+              A.f.nkb * 64);
+  const dim3 grid(kZRegBlocks, 1, L), block(192);
+  switch (mode) {
+    case kFzScale: hipLaunchKernelGGL((fz64_reg<kFzScale, R>), grid, block, lds, s, A); break;
+    case kFzPre: hipLaunchKernelGGL((fz64_reg<kFzPre, R>), grid, block, lds, s, A); break;
+    case kFzInit: hipLaunchKernelGGL((fz64_reg<kFzInit, R>), grid, block, lds, s, A); break;
+    default: hipLaunchKernelGGL((fz64_reg<kFzPCG, R>), grid, block, lds, s, A);
+  }
+  PGJ_LAUNCHED("fz64_reg");
+  return PGJ_OK;
+}
+
 template <int N, typename R>
 int fz_static_n(FFTArgsT<R>& A, int L, int mode, hipStream_t s) {
+  if constexpr (N == 64) {
+    if (A.zwide == kZReg) return fz64_reg_launch<R>(A, L, mode, s);
+  }
   if constexpr (kWideKC<N, R> > 0) {
-    if (A.zwide) return fz_static_kc<N, R, kWideKC<N, R>>(A, L, mode, s);
+    if (A.zwide == 1) return fz_static_kc<N, R, kWideKC<N, R>>(A, L, mode, s);
   }
   return fz_static_kc<N, R, 0>(A, L, mode, s);
 }
@@ -667,6 +853,11 @@ bool zpass_wide(const pgj_grid& g, int L, int es) {
   using Zn = ZPass<64, double>;
   const int64_t narrow = int64_t(Zn::NKB) * 64 * L;  // blocks of the default pass
   return cus > 0 && narrow > cus * Zn::per_cu;
+}
+
+int zpass_variant(const pgj_grid& g, int L, int es) {
+  if (static_cube(g) && g.nx == 64 && fft_plan(64, es, 1) == 2) return kZReg;
+  return zpass_wide(g, L, es) ? 1 : 0;
 }
 
 }  // namespace pgj

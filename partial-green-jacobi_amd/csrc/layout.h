@@ -188,14 +188,15 @@ inline FFTGeom fft_geom(const pgj_grid& g) {
   int kc = 1;
   while (kc < f.nkx && size_t(kc + 1) * col + 64 * 8 <= 56 * 1024) ++kc;
   if (g.nx == g.ny && g.ny == g.nz) {
-    // cubes: at least as many chunks as the compile-time kernels use (fft_static.hip, ZPass<N, R>::NKB), so the
-    // per-block partials laid out with nkb below hold either kind of z pass
+    // cubes: at least as many chunks as the compile-time kernels use (fft_static.hip, ZPass<N, R>::NKB; at 64^3
+    // 5 x 64 rows for the 264 blocks of the register-resident pass), so the per-block partials laid out with
+    // nkb below hold every kind of z pass
     switch (g.nx) {
       case 16: kc = 9; break;
       case 24: kc = 13; break;
       case 32: kc = 17; break;
       case 48: kc = 5; break;
-      case 64: kc = 11; break;
+      case 64: kc = 7; break;
       case 96: kc = 7; break;
       case 128: kc = 5; break;
       default: break;

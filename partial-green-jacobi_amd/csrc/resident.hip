@@ -213,7 +213,7 @@ int resident_init_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgs
   A.st = state_view(state, L, l0);
   A.soa = 1;
   A.group = group;
-  A.zwide = zpass_wide(*g, L, int(sizeof(T)));  // from the workspace's load count: every slice agrees
+  A.zwide = zpass_variant(*g, L, int(sizeof(T)));  // from the workspace's load count: every slice agrees
   if (int e = fxy(A, *g, nl, kFxyPre, s)) return e;
   if (int e = fz(A, *g, nl, kFzInit, s)) return e;
   return fxyi(A, *g, nl, s);
@@ -243,7 +243,7 @@ int resident_stage_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArg
   F.st = st;
   F.soa = 1;
   F.group = group;
-  F.zwide = zpass_wide(*g, L, int(sizeof(T)));
+  F.zwide = zpass_variant(*g, L, int(sizeof(T)));
   if (stage == 0) {
     RowArgsT<T> K{};
     K.a = a + sl.design() * sl.N; K.u = p + v; K.out = q + v; K.st = st;

@@ -44,13 +44,14 @@ def tune_row_march(grid, L, es, launch_hex8, reps=6, candidates=None):
 
 
 def tune_fft(grid, es, launch_stage, reps=6):
-    """Radix plan (0: radix 4 first, 1: radix 8 first) of each FFT pass, by timing ``launch_stage(k)`` for
-    stages 1 (forward xy), 2 (z and Green multiply) and 3 (inverse xy); recorded and returned."""
+    """Radix plan (0: radix 4 first, 1: radix 8 first; for the z pass at 64^3 also 2: the transforms in
+    registers) of each FFT pass, by timing ``launch_stage(k)`` for stages 1 (forward xy), 2 (z and Green
+    multiply) and 3 (inverse xy); recorded and returned."""
     lib = _lib.lib()
     out = {}
     for pas, stage in enumerate((STAGE_FXY, STAGE_FZ, STAGE_FXYI)):
         times = {}
-        for plan in (0, 1):
+        for plan in ((0, 1, 2) if pas == 1 and grid.nx == 64 else (0, 1)):
             _lib.check(lib.pgj_fft_set_plan(grid, es, pas, plan))
             times[plan] = time_launch(lambda: launch_stage(stage), reps)
         plan = min(times, key=times.get)
@@ -91,6 +92,6 @@ def reset(grid, L=None, es=8):
     """Default shapes and plans again (row-march shape for L loads per launch, when given)."""
     lib = _lib.lib()
     for pas in range(3):
-        _lib.check(lib.pgj_fft_set_plan(grid, es, pas, 0))
+        _lib.check(lib.pgj_fft_set_plan(grid, es, pas, -1))
     if L is not None:
         _lib.check(lib.pgj_hex8_set_shape(grid, L, es, 0, 0, 0, 0))
