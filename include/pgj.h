@@ -135,7 +135,7 @@ int pgj_hex8_set_shape(const pgj_grid* g, int L, int es, int R, int W, int ZC, i
  * plan 1 radix-8 stages first (64 = 8 x 8); for the z pass at 64^3, plan 2
  * keeps the transforms in registers (two radix-8 passes with one LDS transpose
  * each, LDS otherwise only for the 3x3 Green multiply). plan -1 restores the
- * default: plan 0, except plan 2 for the FP32 z pass at 64^3. The
+ * default, plan 0. The
  * choice holds for every later launch of that pass on any n^3 grid of that
  * precision (a graph keeps the kernels of its capture; a solver reads the plan
  * when it sets up its launches). get_plan writes the three passes' plans.

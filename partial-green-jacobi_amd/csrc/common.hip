@@ -37,14 +37,10 @@ std::mutex g_plan_mutex;
 std::map<PlanKey, int> g_plans;
 }  // namespace
 
-// Default plans: radix 4 first everywhere, except the FP32 z pass at 64^3, which defaults to the
-// register-resident pass (2.3x faster than either radix plan at 64^3 x 6 loads; in FP64 it is slower).
-static int default_plan(int n, int es, int pass) { return n == 64 && es == 4 && pass == 1 ? 2 : 0; }
-
 int fft_plan(int n, int es, int pass) {
   std::lock_guard<std::mutex> lock(g_plan_mutex);
   auto it = g_plans.find(PlanKey(n, es, pass));
-  return it == g_plans.end() ? default_plan(n, es, pass) : it->second;
+  return it == g_plans.end() ? 0 : it->second;
 }
 
 bool tuned_shape(const pgj_grid& g, int L, int es, int* R, int* W, int* ZC, int* plane) {
