@@ -89,6 +89,22 @@ def stage_bytes(L, n):
     }
 
 
+def copy_us(nbytes, reps=20):
+    """Device time of a plain device-to-device copy moving ``nbytes`` (half read, half written), back to back on
+    the same buffers as the stage probes are: the size-matched bandwidth ceiling of one launch."""
+    n = max(1, int(nbytes // 16))
+    src = torch.ones(n, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e3
+
+
 def probe_stages(res, iters, nl=None):
     """Average device duration of each stage with all loads active, for launches over the load slice [0, nl) of
     the set (the solve's launch shape: with the two-stream split each launch covers half the group): after one
@@ -939,6 +955,11 @@ def main():
     nbytes = stage_bytes(nl, n)
     stages = {k: {'ms': durations[k], 'bytes': nbytes[k], 'GBs': nbytes[k] / durations[k] / 1e6,
                   'frac': nbytes[k] / durations[k] / 1e6 / HBM_PEAK_GBS} for k in durations}
+    for k in stages:
+        # a plain copy of the same bytes (repeated buffers under the 256 MiB Infinity Cache, as the probe's)
+        c = copy_us(nbytes[k])
+        stages[k]['copy_us'] = c
+        stages[k]['frac_of_copy'] = c / (1e3 * durations[k])
     # the four stages run once per iteration each and take 20-23 % of the solve's device time apiece (kernel
     # trace), so the roofline reports the one furthest below the HBM peak
     dom = min(stages, key=lambda k: stages[k]['GBs'])
