@@ -455,6 +455,33 @@ size_t pgj_hybrid_stress_ws_bytes(const pgj_grid* g);
 int pgj_hybrid_avg_stress(const pgj_grid* g, const double* pool, const int32_t* index, const double* u,
                           const double macro[6], double* ws, double* out6, void* stream);
 
+/* Resident PCG over the general-C21 operator: one design of L loads (group = L)
+ * with the resident state, workspace (pgj_resident_state_bytes,
+ * pgj_resident_ws_bytes), first direction (pgj_resident_init) and FFT stages of
+ * the scalar solver. The grid's lam, mu are the Green reference medium (the
+ * zeman medium of the pool); s/s_hat scale the preconditioner as in the scalar
+ * solver (pass the unit scale for M = P G P). The reference solves this
+ * operator with the generic PCG inside compute_effective_stiffness_batched
+ * (solver/src/runtime/fi/source/femlib/batched/driver.py:13-75, pcg.py:20-252);
+ * these run it as the resident recurrence with certification
+ * (exp/asg_gpu_resident_20261103/resident.py:39-58, :186-209). SoA vectors.
+ * setup: rhs_l = pgj_hybrid_rhs of the L macro strains (host L*6), norms,
+ * targets and the iteration cap, as pgj_resident_setup. */
+int pgj_hybrid_resident_setup(const pgj_grid* g, int L, int l0, int nl, const double* pool, const int32_t* index,
+                              const double* macros, double rtol, int64_t cap, double* rhs, void* state, void* ws,
+                              void* stream);
+/* u <- P u; r = rhs - K u by the gather apply; absolute = |r| (pgj_resident_certify). */
+int pgj_hybrid_resident_certify(const pgj_grid* g, int L, int l0, int nl, const double* pool, const int32_t* index,
+                                double* u, const double* rhs, double* r, void* state, void* ws, void* stream);
+/* Stage 0 is q = K p by the tiled apply (p.q partials per block); stages 1-3
+ * and the iteration loop as pgj_resident_stage / pgj_resident_iterate. */
+int pgj_hybrid_resident_stage(const pgj_grid* g, int L, int l0, int nl, const void* plan, const double* pool,
+                              const int32_t* index, const double* s, const void* s_hat, double* u, double* r,
+                              double* p, double* p_next, double* q, void* state, void* ws, int stage, void* stream);
+int pgj_hybrid_resident_iterate(const pgj_grid* g, int L, int l0, int nl, const void* plan, const double* pool,
+                                const int32_t* index, const double* s, const void* s_hat, double* u, double* r,
+                                double* p0, double* p1, double* q, void* state, void* ws, int iters, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

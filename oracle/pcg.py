@@ -129,6 +129,37 @@ class ResidentPCG:
         return u, r, p, newrz, active, it, bad
 
 
+class HybridResidentPCG(ResidentPCG):
+    """``ResidentPCG`` over a general-C21 material (``oracle.hybrid``): K and the rhs from the pool rows
+    (``index`` None: row e for element e), the Green preconditioner of the isotropic medium ``lam_mu`` (the
+    operator's reference medium) without scaling. The recurrence, certification and restarts are the
+    resident solver's (``resident.py:39-58``, ``:186-209``)."""
+
+    def __init__(self, pool, index, grid, voxel, loads, batch=1, lam_mu=None, chunk=4):
+        from . import hybrid
+        self._hybrid = hybrid
+        self.pool = np.asarray(pool, float).reshape(-1, 21)
+        self.index = None if index is None else np.asarray(index).reshape(-1)
+        self.grid = tuple(int(v) for v in grid)
+        self.n = self.grid[2]
+        self.voxel = tuple(float(v) for v in voxel)
+        self.G6 = green.symbol(self.grid, self.voxel, *lam_mu)
+        self.loads = np.atleast_2d(np.asarray(loads, float))
+        self.batch = batch
+        self.s = None
+        self.chunk = chunk
+        self.solutions = np.zeros((len(self.loads), 3 * int(np.prod(self.grid))))
+
+    def K(self, p, active):
+        return self._hybrid.apply(self.pool, self.index, p, self.grid, self.voxel, active)
+
+    def true_K(self, u):
+        return self._hybrid.apply(self.pool, self.index, u, self.grid, self.voxel)
+
+    def rhs(self, macro):
+        return self._hybrid.rhs(self.pool, self.index, macro, self.grid, self.voxel)
+
+
 def solve_pcg_batched(apply, apply_precond, rhs, *, tol=1e-6, max_iter=2000, x0=None,
                       update_mode='eager', convergence_check_stride=1, adaptive_factor=None):
     """Best-iterate batched PCG (``pcg.py:20-252``); returns (best_u, n_iter, rel_res).

@@ -58,4 +58,17 @@ int launch_rhs(const pgj_grid& g, const double* a, const double* macro, double* 
 int launch_scale(const pgj_grid& g, const double* a, double alpha, double* d, double* s_out, double* ws, int soa,
                  hipStream_t s, const double* alpha_dev = nullptr);
 
+// General-C21 operator (hybrid.hip) in the resident solver: SoA vectors throughout.
+// Blocks per load of the PCG-mode tiled apply over L loads (its p.q partial count).
+int c21_pcg_blocks(const pgj_grid& g, int L, int* blocks);
+// q = K p for the active loads of `active`, p.q partials per block into ppq[l * ppq_stride + block].
+int launch_c21_pcg(const pgj_grid& g, int L, const double* pool, const int32_t* index, const double* p,
+                   const uint8_t* active, double* q, double* ppq, int ppq_stride, hipStream_t s);
+// r = rhs - K u, per-block |r|^2 into slot 3 of the 4-wide partial rows [l][block][4] (256-node blocks).
+int launch_c21_certify(const pgj_grid& g, int L, const double* pool, const int32_t* index, const double* u,
+                       const double* rhs, double* r, double* pres, hipStream_t s);
+// rhs of one macro strain (host 6 doubles), AoS or SoA (soa != 0).
+int launch_c21_rhs(const pgj_grid& g, const double* pool, const int32_t* index, const double* macro, double* out,
+                   int soa, hipStream_t s);
+
 }  // namespace pgj

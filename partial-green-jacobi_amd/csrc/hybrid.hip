@@ -24,15 +24,20 @@ __device__ __forceinline__ const double* c21_row(const double* pool, const int32
   return pool + (index ? int64_t(index[e]) : e) * 21;
 }
 
-// q_l = sum_e P_e^T Ke(C_e) P_e u_l; AoS (L, 3N); inactive loads give zeros.
+// q_l = sum_e P_e^T Ke(C_e) P_e u_l; AoS (L, 3N), or SoA (L, 3, N) with SOA; inactive loads give zeros.
+// CERT: out = rhs - K u and per-block |out|^2 into slot 3 of the 4-wide partial rows [l][block][4].
+template <bool SOA, bool CERT>
 __global__ __launch_bounds__(256) void c21_apply_kernel(int nz, int ny, int nx, ElementCoef coef, const double* pool,
                                                         const int32_t* index, const double* u, const uint8_t* active,
-                                                        double* out) {
+                                                        double* out, const double* rhs, double* pres) {
+  __shared__ double red[4];
   const int l = blockIdx.y;
   const int64_t N = int64_t(nx) * ny * nz;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= N) return;
   const int64_t base = int64_t(l) * 3 * N;
+  auto at = [&](int64_t node, int d) { return SOA ? base + int64_t(d) * N + node : base + 3 * node + d; };
+  double res2 = 0.0;
+  if (i < N) {
   double acc[3] = {0.0, 0.0, 0.0};
   if (!active || active[l]) {
     const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (int64_t(nx) * ny));
@@ -46,7 +51,7 @@ __global__ __launch_bounds__(256) void c21_apply_kernel(int nz, int ny, int nx, 
         const int64_t node = (int64_t(wrapi(ez + ((n >> 2) & 1), nz)) * ny + wrapi(ey + ((n >> 1) & 1), ny)) * nx +
                              wrapi(ex + (n & 1), nx);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[base + 3 * node + d];
+        for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[at(node, d)];
       }
       unpack_c21(c21_row(pool, index, e), C);
       hex8_element_c(ue, C, coef, fe);
@@ -60,7 +65,20 @@ __global__ __launch_bounds__(256) void c21_apply_kernel(int nz, int ny, int nx, 
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) out[base + 3 * i + d] = acc[d];
+  for (int d = 0; d < 3; ++d) {
+    if (CERT) {
+      const double r = rhs[at(i, d)] - acc[d];
+      out[at(i, d)] = r;
+      res2 = fma(r, r, res2);
+    } else {
+      out[at(i, d)] = acc[d];
+    }
+  }
+  }
+  if (CERT) {
+    const double t = block_sum(res2, red);
+    if (threadIdx.x == 0) pres[(int64_t(l) * gridDim.x + blockIdx.x) * 4 + 3] = t;
+  }
 }
 
 // Tiled apply: a block owns a 16 x 16 column of nodes over a z-chunk and marches
@@ -81,12 +99,16 @@ __device__ __forceinline__ int modi(int v, int n) {
   return r < 0 ? r + n : r;
 }
 
+// SOA: vectors (L, 3, N) instead of (L, N, 3). PCG: the resident iteration's q = K p, where an inactive load
+// writes nothing and each block leaves the p.q partial of its owned nodes in ppq[l * ppq_stride + block].
+template <bool SOA, bool PCG>
 __global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, int nx, int zc, ElementCoef coef,
                                                                 const double* pool, const int32_t* index,
                                                                 const double* u, const uint8_t* active,
-                                                                double* out) {
+                                                                double* out, double* ppq, int ppq_stride) {
   __shared__ double np[2][kSide * kSide * 3];  // node planes z and z+1 (by parity)
   __shared__ double fx[2][kElem * kElem * 9];
+  __shared__ double red[kTileThreads / 64];
   const int l = blockIdx.z;
   const int tiles_x = (nx + kTile - 1) / kTile;
   const int x0 = int(blockIdx.x % tiles_x) * kTile, y0 = int(blockIdx.x / tiles_x) * kTile;
@@ -94,6 +116,7 @@ __global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, 
   const int64_t N = int64_t(nx) * ny * nz;
   const double* ul = u + int64_t(l) * 3 * N;
   double* ol = out + int64_t(l) * 3 * N;
+  auto at = [&](int64_t node, int d) { return SOA ? int64_t(d) * N + node : 3 * node + d; };
   const int t = threadIdx.x;
   int i = 0, j = 0;
   bool has = true;
@@ -114,10 +137,10 @@ __global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, 
   const int gx = modi(x0 + i, nx), gy = modi(y0 + j, ny);
 
   if (active && !active[l]) {
-    if (owner)
+    if (owner && !PCG)
       for (int z = z0; z < zend; ++z) {
         const int64_t node = (int64_t(z) * ny + gy) * nx + gx;
-        for (int d = 0; d < 3; ++d) ol[3 * node + d] = 0.0;
+        for (int d = 0; d < 3; ++d) ol[at(node, d)] = 0.0;
       }
     return;
   }
@@ -132,7 +155,7 @@ __global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, 
       if (k < kPlane) {
         const int64_t node = (int64_t(pz) * ny + modi(y0 - 1 + k / kSide, ny)) * nx + modi(x0 - 1 + k % kSide, nx);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) v[s][d] = ul[3 * node + d];
+        for (int d = 0; d < 3; ++d) v[s][d] = ul[at(node, d)];
       }
     }
   };
@@ -166,15 +189,19 @@ __global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, 
 
   const int slot = (j + 1) * kElem + (i + 1);
   double carry[3] = {0.0, 0.0, 0.0};
+  double pq = 0.0;
 #pragma unroll 1
   for (int z = z0 - 1; z < zend; ++z) {
     const bool more = z + 1 < zend;
     if (more) fetch(z + 2, pre);
-    double fe[24];
+    double fe[24], pown[3] = {0.0, 0.0, 0.0};
     if (has) {
       double ue[24], C[6][6];
       read_nodes(z, ue);
       read_nodes(z + 1, ue + 12);
+      // local node 0 of the element is the node this thread owns (its input value at plane z)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pown[d] = ue[d];
       const int64_t e = (int64_t(modi(z, nz)) * ny + gy) * nx + gx;
       unpack_c21(c21_row(pool, index, e), C);
       hex8_element_c(ue, C, coef, fe);
@@ -198,17 +225,24 @@ __global__ __launch_bounds__(kTileThreads) void c21_tile_kernel(int nz, int ny, 
       if (z >= z0) {
         const int64_t node = (int64_t(z) * ny + gy) * nx + gx;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) ol[3 * node + d] = bottom[d];
+        for (int d = 0; d < 3; ++d) {
+          ol[at(node, d)] = bottom[d];
+          if (PCG) pq = fma(pown[d], bottom[d], pq);
+        }
       }
     }
     if (more) stage(z + 2, pre);  // over plane z, which every thread has read before the barrier
     __syncthreads();
   }
+  if (PCG) {
+    const double total = block_sum(pq, red);
+    if (t == 0) ppq[int64_t(l) * ppq_stride + blockIdx.y * gridDim.x + blockIdx.x] = total;
+  }
 }
 
 // f = -sum_e V Bbar_e^T C_e E (the element force of the affine field of E), AoS (1, 3N).
 __global__ __launch_bounds__(256) void c21_rhs_kernel(int nz, int ny, int nx, ElementCoef coef, const double* pool,
-                                                      const int32_t* index, Strain6 E, double* out) {
+                                                      const int32_t* index, Strain6 E, double* out, int soa) {
   const int64_t N = int64_t(nx) * ny * nz;
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -232,7 +266,7 @@ __global__ __launch_bounds__(256) void c21_rhs_kernel(int nz, int ny, int nx, El
     f[1] -= dx * s[5] + dy * s[1] + dz * s[3];
     f[2] -= dx * s[4] + dy * s[3] + dz * s[2];
   }
-  for (int d = 0; d < 3; ++d) out[3 * i + d] = f[d];
+  for (int d = 0; d < 3; ++d) out[soa ? int64_t(d) * N + i : 3 * i + d] = f[d];
 }
 
 // Per-block sums of C_e (E + Bbar u_e) for the mean stress.
@@ -285,8 +319,11 @@ __global__ void c21_stress_finish(const double* part, int nblk, int64_t N, doubl
 
 using namespace pgj;
 
+namespace pgj {
 namespace {
-// Blocks of the tiled apply resident at once on the current device (CUs x blocks per CU), cached per device.
+// Blocks of the tiled apply resident at once on the current device (CUs x blocks per CU), cached per device and
+// kernel instance.
+template <bool SOA, bool PCG>
 int tile_slots(int* out) {
   static int cache[64] = {};
   int dev = 0;
@@ -295,7 +332,8 @@ int tile_slots(int* out) {
   if (e == hipSuccess && cache[dev] == 0) {
     int cus = 0, per_cu = 0;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c21_tile_kernel, kTileThreads, 0);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c21_tile_kernel<SOA, PCG>, kTileThreads, 0);
     if (e == hipSuccess) cache[dev] = std::max(1, cus * std::max(1, per_cu));
   }
   if (e != hipSuccess) {
@@ -303,6 +341,38 @@ int tile_slots(int* out) {
     return PGJ_ERR_HIP;
   }
   *out = cache[dev];
+  return PGJ_OK;
+}
+
+int tiles_of(const pgj_grid& g) { return ((g.nx + kTile - 1) / kTile) * ((g.ny + kTile - 1) / kTile); }
+
+// z-chunk with the fewest layer steps on the critical path: rounds of resident blocks x (chunk + halo layer)
+template <bool SOA, bool PCG>
+int tile_zc(const pgj_grid& g, int L, int* zc_out) {
+  int slots = 0;
+  if (int e = tile_slots<SOA, PCG>(&slots)) return e;
+  int zc = 2;
+  int64_t best = INT64_MAX;
+  for (int c = 2; c <= 32; c *= 2) {
+    const int64_t blocks = int64_t(tiles_of(g)) * L * ((g.nz + c - 1) / c);
+    const int64_t cost = ((blocks + slots - 1) / slots) * (std::min(c, g.nz) + 1);
+    if (cost < best) {
+      best = cost;
+      zc = c;
+    }
+  }
+  *zc_out = zc;
+  return PGJ_OK;
+}
+
+template <bool SOA, bool PCG>
+int launch_tile(const pgj_grid& g, int L, const double* pool, const int32_t* index, const double* u,
+                const uint8_t* active, double* out, double* ppq, int ppq_stride, hipStream_t s) {
+  int zc = 0;
+  if (int e = tile_zc<SOA, PCG>(g, L, &zc)) return e;
+  hipLaunchKernelGGL((c21_tile_kernel<SOA, PCG>), dim3(tiles_of(g), (g.nz + zc - 1) / zc, L), dim3(kTileThreads), 0,
+                     s, g.nz, g.ny, g.nx, zc, coef_of(g), pool, index, u, active, out, ppq, ppq_stride);
+  PGJ_LAUNCHED("c21_tile_kernel");
   return PGJ_OK;
 }
 
@@ -314,36 +384,48 @@ Strain6 macro_of(const double* m) {
 }
 }  // namespace
 
+int c21_pcg_blocks(const pgj_grid& g, int L, int* blocks) {
+  int zc = 0;
+  if (int e = tile_zc<true, true>(g, L, &zc)) return e;
+  *blocks = tiles_of(g) * ((g.nz + zc - 1) / zc);
+  return PGJ_OK;
+}
+
+int launch_c21_pcg(const pgj_grid& g, int L, const double* pool, const int32_t* index, const double* p,
+                   const uint8_t* active, double* q, double* ppq, int ppq_stride, hipStream_t s) {
+  return launch_tile<true, true>(g, L, pool, index, p, active, q, ppq, ppq_stride, s);
+}
+
+int launch_c21_certify(const pgj_grid& g, int L, const double* pool, const int32_t* index, const double* u,
+                       const double* rhs, double* r, double* pres, hipStream_t s) {
+  hipLaunchKernelGGL((c21_apply_kernel<true, true>), dim3(c21_blocks(g), L), dim3(256), 0, s, g.nz, g.ny, g.nx,
+                     coef_of(g), pool, index, u, nullptr, r, rhs, pres);
+  PGJ_LAUNCHED("c21_apply_kernel");
+  return PGJ_OK;
+}
+
+int launch_c21_rhs(const pgj_grid& g, const double* pool, const int32_t* index, const double* macro, double* out,
+                   int soa, hipStream_t s) {
+  hipLaunchKernelGGL(c21_rhs_kernel, dim3(c21_blocks(g)), dim3(256), 0, s, g.nz, g.ny, g.nx, coef_of(g), pool, index,
+                     macro_of(macro), out, soa);
+  PGJ_LAUNCHED("c21_rhs_kernel");
+  return PGJ_OK;
+}
+}  // namespace pgj
+
 extern "C" int pgj_hybrid_apply(const pgj_grid* g, int L, const double* pool, const int32_t* index, const double* u,
                                 const uint8_t* active, double* out, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(L >= 1 && L <= 65535 && pool && u && out, "invalid hybrid apply arguments");
-  const int tiles = ((g->nx + kTile - 1) / kTile) * ((g->ny + kTile - 1) / kTile);
-  // z-chunk with the fewest layer steps on the critical path: rounds of resident blocks x (chunk + halo layer)
-  int slots = 0;
-  if (int e = tile_slots(&slots)) return e;
-  int zc = 2;
-  int64_t best = INT64_MAX;
-  for (int c = 2; c <= 32; c *= 2) {
-    const int64_t blocks = int64_t(tiles) * L * ((g->nz + c - 1) / c);
-    const int64_t cost = ((blocks + slots - 1) / slots) * (std::min(c, g->nz) + 1);
-    if (cost < best) {
-      best = cost;
-      zc = c;
-    }
-  }
-  hipLaunchKernelGGL(c21_tile_kernel, dim3(tiles, (g->nz + zc - 1) / zc, L), dim3(kTileThreads), 0,
-                     as_stream(stream), g->nz, g->ny, g->nx, zc, coef_of(*g), pool, index, u, active, out);
-  PGJ_LAUNCHED("c21_tile_kernel");
-  return PGJ_OK;
+  return launch_tile<false, false>(*g, L, pool, index, u, active, out, nullptr, 0, as_stream(stream));
 }
 
 extern "C" int pgj_hybrid_apply_gather(const pgj_grid* g, int L, const double* pool, const int32_t* index,
                                        const double* u, const uint8_t* active, double* out, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(L >= 1 && L <= 65535 && pool && u && out, "invalid hybrid apply arguments");
-  hipLaunchKernelGGL(c21_apply_kernel, dim3(c21_blocks(*g), L), dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,
-                     coef_of(*g), pool, index, u, active, out);
+  hipLaunchKernelGGL((c21_apply_kernel<false, false>), dim3(c21_blocks(*g), L), dim3(256), 0, as_stream(stream),
+                     g->nz, g->ny, g->nx, coef_of(*g), pool, index, u, active, out, nullptr, nullptr);
   PGJ_LAUNCHED("c21_apply_kernel");
   return PGJ_OK;
 }
@@ -352,25 +434,22 @@ extern "C" int pgj_hybrid_rhs(const pgj_grid* g, const double* pool, const int32
                               double* out, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(pool && macro && out, "invalid hybrid rhs arguments");
-  hipLaunchKernelGGL(c21_rhs_kernel, dim3(c21_blocks(*g)), dim3(256), 0, as_stream(stream), g->nz, g->ny, g->nx,
-                     coef_of(*g), pool, index, macro_of(macro), out);
-  PGJ_LAUNCHED("c21_rhs_kernel");
-  return PGJ_OK;
+  return launch_c21_rhs(*g, pool, index, macro, out, 0, as_stream(stream));
 }
 
 extern "C" size_t pgj_hybrid_stress_ws_bytes(const pgj_grid* g) {
   if (!g) return 0;
-  return align256(size_t(c21_blocks(*g)) * 6 * 8);
+  return align256(size_t(pgj::c21_blocks(*g)) * 6 * 8);
 }
 
 extern "C" int pgj_hybrid_avg_stress(const pgj_grid* g, const double* pool, const int32_t* index, const double* u,
                                      const double macro[6], double* ws, double* out6, void* stream) {
   if (int e = check_grid(g)) return e;
   PGJ_REQUIRE(pool && u && macro && ws && out6, "invalid hybrid stress arguments");
-  const int nblk = c21_blocks(*g);
+  const int nblk = pgj::c21_blocks(*g);
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(c21_stress_partials, dim3(nblk), dim3(256), 0, s, g->nz, g->ny, g->nx, coef_of(*g), pool, index, u,
-                     macro_of(macro), ws);
+                     pgj::macro_of(macro), ws);
   PGJ_LAUNCHED("c21_stress_partials");
   hipLaunchKernelGGL(c21_stress_finish, dim3(6), dim3(64), 0, s, ws, nblk, int64_t(g->nx) * g->ny * g->nz, out6);
   PGJ_LAUNCHED("c21_stress_finish");

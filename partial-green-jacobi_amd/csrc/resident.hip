@@ -222,16 +222,22 @@ int resident_init_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgs
 // One of the four fused iteration stages on a slice (precision T). The
 // direction of this iteration is p (read by stages 0, 1, 3); stage 3 writes
 // the next direction to p_next.
+// The operator of stage 0: the scalar SIMP field a (the row-march), or a general-C21 pool (the tiled apply).
+struct C21Operand {
+  const double* pool;
+  const int32_t* index;
+};
+
 template <typename T>
 int resident_stage_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArgsT<T> F, const T* a, const T* sc,
                      const cplx<T>* s_hat, T* u, T* r, T* p, T* p_next, T* q, void* state, void* ws, int stage,
-                     hipStream_t s) {
+                     hipStream_t s, const C21Operand* c21 = nullptr) {
   Slice sl{};
   if (int e = make_slice(g, L, group, l0, nl, &sl)) return e;
-  PGJ_REQUIRE(a && u && r && p && p_next && q && state && ws, "invalid stage arguments");
+  PGJ_REQUIRE((a || c21) && u && r && p && p_next && q && state && ws, "invalid stage arguments");
   PGJ_REQUIRE(stage >= 0 && stage <= 3, "stage must be 0..3, got %d", stage);
   PGJ_REQUIRE((sc == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
-  PGJ_REQUIRE(rowmarch_config(*g, nl).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
+  PGJ_REQUIRE(c21 || rowmarch_config(*g, nl).valid, "resident iteration needs nx <= 128 with nx %% ceil(nx/64) == 0");
   ResidentBuffersT<T> b = resident_buffers<T>(*g, L, ws, l0);
   StateView st = state_view(state, L, l0);
   const int64_t v = sl.vec();
@@ -240,11 +246,21 @@ int resident_stage_t(const pgj_grid* g, int L, int group, int l0, int nl, FFTArg
   F.shat = s_hat ? s_hat + sl.design() * 3 * sl.nf : nullptr;
   F.spec = b.spec; F.pr = b.pr; F.pg = b.pg; F.zout = p_next + v;
   F.ppq = b.ppq; F.ppq_count = hex8_blocks_per_load(*g, nl, int(sizeof(T))); F.ppq_stride = hex8_blocks_bound(*g);
+  if constexpr (sizeof(T) == 8) {
+    if (c21) {
+      if (int e = c21_pcg_blocks(*g, nl, &F.ppq_count)) return e;
+      PGJ_REQUIRE(F.ppq_count <= F.ppq_stride, "tiled apply blocks (%d) exceed the partial rows (%d)", F.ppq_count,
+                  F.ppq_stride);
+    }
+  }
   F.st = st;
   F.soa = 1;
   F.group = group;
   F.zwide = zpass_variant(*g, L, int(sizeof(T)));
   if (stage == 0) {
+    if constexpr (sizeof(T) == 8) {
+      if (c21) return launch_c21_pcg(*g, nl, c21->pool, c21->index, p + v, st.active, q + v, b.ppq, F.ppq_stride, s);
+    }
     RowArgsT<T> K{};
     K.a = a + sl.design() * sl.N; K.u = p + v; K.out = q + v; K.st = st;
     K.ppq = b.ppq; K.ppq_stride = F.ppq_stride;
@@ -314,6 +330,79 @@ extern "C" int pgj_resident_iterate(const pgj_grid* g, int L, int group, int l0,
     for (int stage = 0; stage < 4; ++stage)
       if (int e = pgj_resident_stage(g, L, group, l0, nl, plan, a, sc, s_hat, u, r, p, p_next, q, state, ws,
                                      stage, stream))
+        return e;
+  }
+  return PGJ_OK;
+}
+
+// ------------------------------------------ general-C21 operator (FP64)
+// The resident solver over a general-C21 material (one design: group = L): the tiled apply in PCG mode as
+// stage 0, the same FFT stages, and the gather apply for set-up and certification.
+
+extern "C" int pgj_hybrid_resident_setup(const pgj_grid* g, int L, int l0, int nl, const double* pool,
+                                         const int32_t* index, const double* macros, double rtol, int64_t cap,
+                                         double* rhs, void* state, void* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  Slice sl{};
+  if (int e = make_slice(g, L, L, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(pool && macros && rhs && state && ws, "invalid hybrid resident setup arguments");
+  PGJ_REQUIRE(cap >= 1, "iteration cap must be at least 1, got %lld", (long long)cap);
+  const int64_t N = sl.N;
+  hipStream_t s = as_stream(stream);
+  double* f = rhs + sl.vec();
+  for (int l = 0; l < nl; ++l)
+    if (int e = launch_c21_rhs(*g, pool, index, macros + 6 * l, f + int64_t(l) * 3 * N, 1, s)) return e;
+  ResidentBuffers b = resident_buffers<double>(*g, L, ws, l0);
+  const int nblk = b.lay.nblk_cert;
+  hipLaunchKernelGGL(square_partials, dim3(nblk, nl), dim3(256), 0, s, f, N, b.pcert);
+  PGJ_LAUNCHED("square_partials");
+  hipLaunchKernelGGL(setup_finish, dim3(nl), dim3(64), 0, s, b.pcert, nblk, rtol, 1e-13 / sqrt(double(g->nx)),
+                     cap, state_view(state, L, l0));
+  PGJ_LAUNCHED("setup_finish");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_hybrid_resident_certify(const pgj_grid* g, int L, int l0, int nl, const double* pool,
+                                           const int32_t* index, double* u, const double* rhs, double* r,
+                                           void* state, void* ws, void* stream) {
+  if (int e = check_grid(g)) return e;
+  Slice sl{};
+  if (int e = make_slice(g, L, L, l0, nl, &sl)) return e;
+  PGJ_REQUIRE(pool && u && rhs && r && state && ws, "invalid hybrid certify arguments");
+  hipStream_t s = as_stream(stream);
+  ResidentBuffers b = resident_buffers<double>(*g, L, ws, l0);
+  double* us = u + sl.vec();
+  if (int e = launch_component_sums(*g, nl, us, b.pcert, 1, s)) return e;
+  if (int e = launch_subtract_mean(*g, nl, us, b.pcert, 1, s)) return e;
+  if (int e = launch_c21_certify(*g, nl, pool, index, us, rhs + sl.vec(), r + sl.vec(), b.pcert, s)) return e;
+  hipLaunchKernelGGL(certify_finish, dim3(nl), dim3(64), 0, s, b.pcert, b.lay.nblk_cert, state_view(state, L, l0));
+  PGJ_LAUNCHED("certify_finish");
+  return PGJ_OK;
+}
+
+extern "C" int pgj_hybrid_resident_stage(const pgj_grid* g, int L, int l0, int nl, const void* plan,
+                                         const double* pool, const int32_t* index, const double* sc,
+                                         const void* s_hat, double* u, double* r, double* p, double* p_next,
+                                         double* q, void* state, void* ws, int stage, void* stream) {
+  if (int e = check_fft(g)) return e;
+  PGJ_REQUIRE(plan && pool, "invalid hybrid stage arguments");
+  const C21Operand c21{pool, index};
+  return resident_stage_t<double>(g, L, L, l0, nl, fft_args(*g, plan), nullptr, sc,
+                                  static_cast<const double2*>(s_hat), u, r, p, p_next, q, state, ws, stage,
+                                  as_stream(stream), &c21);
+}
+
+extern "C" int pgj_hybrid_resident_iterate(const pgj_grid* g, int L, int l0, int nl, const void* plan,
+                                           const double* pool, const int32_t* index, const double* sc,
+                                           const void* s_hat, double* u, double* r, double* p0, double* p1,
+                                           double* q, void* state, void* ws, int iters, void* stream) {
+  PGJ_REQUIRE(iters >= 2 && iters % 2 == 0, "iteration count must be even and >= 2, got %d", iters);
+  for (int i = 0; i < iters; ++i) {
+    double* p = (i % 2 == 0) ? p0 : p1;
+    double* p_next = (i % 2 == 0) ? p1 : p0;
+    for (int stage = 0; stage < 4; ++stage)
+      if (int e = pgj_hybrid_resident_stage(g, L, l0, nl, plan, pool, index, sc, s_hat, u, r, p, p_next, q, state,
+                                            ws, stage, stream))
         return e;
   }
   return PGJ_OK;

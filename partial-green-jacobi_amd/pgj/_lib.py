@@ -84,6 +84,16 @@ SIGNATURES = {
     'pgj_hybrid_rhs': (c_int, [GP, c_void_p, c_void_p, c_dp, c_void_p, c_void_p]),
     'pgj_hybrid_stress_ws_bytes': (c_size_t, [GP]),
     'pgj_hybrid_avg_stress': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_dp, c_void_p, c_void_p, c_void_p]),
+    'pgj_hybrid_resident_setup': (c_int, [GP, c_int, c_int, c_int, c_void_p, c_void_p, c_dp, c_double, c_int64,
+                                          c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hybrid_resident_certify': (c_int, [GP, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_hybrid_resident_stage': (c_int, [GP, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_int, c_void_p]),
+    'pgj_hybrid_resident_iterate': (c_int, [GP, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_int, c_void_p]),
     'pgj_oc_ws_bytes': (c_size_t, [c_int]),
     'pgj_cone_filter': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     'pgj_cone_project': (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,

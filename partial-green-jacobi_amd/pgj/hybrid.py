@@ -210,3 +210,169 @@ def enable_fi_reference(op, precision='mixed', layout='upper6'):
 
     op.apply_precond = MethodType(apply, op)
     return op
+
+
+class HybridResident:
+    """The resident multi-load PCG over a general-C21 operator: ``op`` is a ``HybridFIOperator`` with
+    ``enable_fi_reference`` applied (its grid carries the Green reference medium and its plan).
+
+    The recurrence, masks, iteration cap, restarts and true-residual certification are those of
+    ``pgj.Resident`` (``exp/asg_gpu_resident_20261103/resident.py:39-58``, ``:186-209``). The preconditioner is
+    the operator's unscaled FI Green operator, as ``apply_precond``. The loads of a group advance together in
+    hipGraph-captured chunks of ``chunk`` iterations:
+    - ``q = K p`` is the tiled C21 apply with fused p.q partials;
+    - the three FFT passes are those of the scalar solver;
+    - certification uses the gather apply.
+    In the parallel load mode (the default, one group of all loads) the group runs as two halves on two
+    streams.
+
+    This is the resident counterpart of ``compute_effective_stiffness_batched`` over this operator
+    (``solver/src/runtime/fi/source/femlib/batched/driver.py:13-75``), which solves one load at a time with
+    the generic PCG. ``effective_tensor`` gives C^H from the mean stresses of the six solutions.
+    """
+
+    def __init__(self, op, loads=None, load_mode='parallel', batch=None, chunk=4, streams=2, cap=None):
+        from .certify import CAP
+        from ._chunks import MaskPipeline, StreamSplit
+        if not getattr(op, '_fft_enabled', False) or getattr(op, '_plan', None) is None:
+            raise ValueError('enable_fi_reference(op) first: the solver uses its Green plan')
+        if load_mode not in ('serial', 'parallel'):
+            raise ValueError('load_mode must be serial or parallel')
+        if chunk < 2 or chunk % 2:
+            raise ValueError('graph chunk must be even and >= 2')
+        self.op = op
+        self.loads = np.eye(6) if loads is None else np.atleast_2d(np.asarray(loads, dtype=np.float64))
+        nl = len(self.loads)
+        self.batch = 1 if load_mode == 'serial' else (batch or nl)
+        if nl % self.batch:
+            raise ValueError('batch must divide the number of loads')
+        self.cap = CAP if cap is None else int(cap)
+        if self.cap < 1:
+            raise ValueError('iteration cap must be at least 1')
+        self.load_mode, self.chunk = load_mode, chunk
+        m = op.mesh
+        N, B, dev = m.n_nodes, self.batch, op.device
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        self.u, self.r, self.p0, self.p1, self.q, self.rhs = (z(B, 3 * N) for _ in range(6))
+        self.solutions = z(nl, 3 * N)
+        lib = _lib.lib()
+        g = op.grid
+        self.state = torch.zeros(lib.pgj_resident_state_bytes(B), dtype=torch.uint8, device=dev)
+        offs = (_lib.c_size_t * 7)()
+        _lib.check(lib.pgj_resident_state_offsets(B, offs))
+        o = list(offs)
+        self.target2 = self.state[o[1]:o[1] + 8 * B].view(torch.float64)
+        self.norm = self.state[o[2]:o[2] + 8 * B].view(torch.float64)
+        self.absolute = self.state[o[3]:o[3] + 8 * B].view(torch.float64)
+        self.iterations = self.state[o[4]:o[4] + 8 * B].view(torch.int64)
+        self.active = self.state[o[5]:o[5] + B]
+        self.bad = self.state[o[6]:o[6] + B]
+        self.ws = torch.empty(lib.pgj_resident_ws_bytes(g, B), dtype=torch.uint8, device=dev)
+        self._pipeline = MaskPipeline([self.active])
+        self._halves = StreamSplit(B, B, streams)
+        self._pool_args = op._pool()
+        self.init_graph = self.iter_graph = self.cert_graph = None
+
+    def _soa(self, src, dst, L):
+        _lib.call('pgj_aos_to_soa', self.op.grid, L, _lib.ptr(src), _lib.ptr(dst), _lib.stream())
+
+    def _aos(self, src, dst, L):
+        _lib.call('pgj_soa_to_aos', self.op.grid, L, _lib.ptr(src), _lib.ptr(dst), _lib.stream())
+
+    def _setup(self, load_indices, rtol):
+        m = self.loads[list(load_indices)].reshape(-1)
+        nl = len(load_indices)
+        _lib.call('pgj_hybrid_resident_setup', self.op.grid, nl, 0, nl, *self._pool_args, _lib.host_doubles(m),
+                  float(rtol), int(self.cap), _lib.ptr(self.rhs), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
+
+    def certify(self):
+        """u <- P u; r = f - K u (gather apply); absolute = |r|."""
+        _lib.call('pgj_hybrid_resident_certify', self.op.grid, self.batch, 0, self.batch, *self._pool_args,
+                  _lib.ptr(self.u), _lib.ptr(self.rhs), _lib.ptr(self.r), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
+
+    def initialize(self):
+        """active mask and first direction p0 = G r."""
+        _lib.call('pgj_resident_init', self.op.grid, self.batch, self.batch, 0, self.batch, _lib.ptr(self.op._plan),
+                  None, None, _lib.ptr(self.r), _lib.ptr(self.p0), _lib.ptr(self.state), _lib.ptr(self.ws),
+                  _lib.stream())
+
+    def _iterate_slice(self, l0, nl, count):
+        _lib.call('pgj_hybrid_resident_iterate', self.op.grid, self.batch, l0, nl, _lib.ptr(self.op._plan),
+                  *self._pool_args, None, None, _lib.ptr(self.u), _lib.ptr(self.r), _lib.ptr(self.p0),
+                  _lib.ptr(self.p1), _lib.ptr(self.q), _lib.ptr(self.state), _lib.ptr(self.ws), count,
+                  _lib.stream())
+
+    def iteration(self, count=None):
+        """``count`` masked PCG iterations of the group (its halves on two streams in the parallel mode)."""
+        count = int(count or self.chunk)
+        self._halves.run(lambda l0, nl: self._iterate_slice(l0, nl, count), self.batch)
+
+    def capture(self):
+        """Capture the init / chunk / certify graphs (``resident.py:138-150``)."""
+        self._setup(range(self.batch), 1e-8)
+        self.r.normal_()
+        self.certify()
+        self.initialize()
+        self.iteration()
+        torch.cuda.synchronize()
+        for name, fn in (('init_graph', self.initialize), ('iter_graph', self.iteration),
+                         ('cert_graph', self.certify)):
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                fn()
+            setattr(self, name, graph)
+        torch.cuda.synchronize()
+        self.u.zero_()
+        self.solutions.zero_()
+        return self
+
+    def solve(self, rtol=1e-8):
+        """Solve every load from the current solutions (warm start); returns (infos, host_checks) with the
+        resident solver's info keys. Raises ``CertificationError`` if a load cannot be certified."""
+        from .certify import MAX_RESTARTS, fail, infos as make_infos, outcome
+        if self.iter_graph is None:
+            raise RuntimeError('call capture() first')
+        infos, checks = [], 0
+        for start in range(0, len(self.loads), self.batch):
+            idx = list(range(start, start + self.batch))
+            sol = self.solutions[start:start + self.batch]
+            self._soa(sol, self.u, self.batch)
+            self._setup(idx, rtol)
+            restarts = 0
+            self.cert_graph.replay()
+            while True:
+                self.init_graph.replay()
+                checks += self._pipeline.run(self.iter_graph.replay)
+                self.cert_graph.replay()
+                diag = torch.stack((self.norm, self.absolute, self.target2.sqrt(),
+                                    self.iterations.to(torch.float64), self.bad.to(torch.float64)),
+                                   dim=1).cpu().numpy()
+                ok, stop = outcome(diag, self.cap)
+                if ok.all() or restarts >= MAX_RESTARTS or stop:
+                    break
+                restarts += 1
+            info = make_infos(diag, ok, restarts)
+            self._aos(self.u, sol, self.batch)
+            if not ok.all():
+                fail(info)
+            infos.extend(info)
+        return infos, checks
+
+    def effective_tensor(self):
+        """C^H (6, 6): column j is the mean stress of solution j under unit strain e_j."""
+        if len(self.loads) != 6 or not np.array_equal(self.loads, np.eye(6)):
+            raise ValueError('C^H needs the six unit Voigt strains')
+        cols = [self.op.compute_avg_stress(self.solutions[j:j + 1].contiguous(), np.eye(6)[j])[0] for j in range(6)]
+        return np.stack(cols, axis=1)
+
+
+def compute_effective_stiffness_resident(op, tol=1e-8, *, solver=None):
+    """C^H (1, 6, 6) of a general-C21 operator through ``HybridResident`` (captured on first use; pass a solver
+    to reuse its graphs and warm starts), with the per-load infos: the resident counterpart of
+    ``compute_effective_stiffness_batched`` (``solver/src/runtime/fi/source/femlib/batched/driver.py:13-75``)."""
+    solver = HybridResident(op).capture() if solver is None else solver
+    infos, _ = solver.solve(tol)
+    C = torch.tensor(solver.effective_tensor()[None], dtype=torch.float64)
+    return C, infos

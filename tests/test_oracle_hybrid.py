@@ -190,3 +190,20 @@ def test_global_phases_need_no_orientation_and_errors():
         mat.phase_table({'a': dict(kind='isotropic', E=1, nu=0.3)})
     batched = mat.build_material_field(np.zeros((2, 2, 2, 2), np.int32), _materials(), device='cpu')
     assert batched.shape == (2, 8, 21)
+
+
+def test_hybrid_resident_recurrence_equals_scalar_for_isotropic_rows():
+    """The resident recurrence over C21 rows a_e C0 with the Green medium C0 is the scalar recurrence: same
+    iterations, solutions to rounding."""
+    from oracle import pcg
+    C0 = hex8.isotropic(15 / 26, 5 / 13)
+    a = np.random.default_rng(1).uniform(0.1, 1, (6, 6, 6))
+    pool = a.reshape(-1, 1) * hybrid.pack(C0)[None]
+    loads = np.eye(6)[[0, 3]]
+    ref = pcg.ResidentPCG(a, C0, loads, batch=2)
+    ri, _ = ref.solve(1e-10)
+    hyb = pcg.HybridResidentPCG(pool, None, a.shape, hex8.voxel_of(a.shape), loads, batch=2,
+                                lam_mu=(15 / 26, 5 / 13))
+    hi, _ = hyb.solve(1e-10)
+    assert [i['iterations'] for i in ri] == [i['iterations'] for i in hi]
+    assert np.abs(ref.solutions - hyb.solutions).max() < 1e-14 * np.abs(ref.solutions).max()
