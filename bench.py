@@ -473,7 +473,7 @@ def hybrid_form(args):
         torch.cuda.synchronize()
         return 1e3 * float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
-    # the resident solver over the same operator: six loads in one captured group (two 3-load halves on two
+    # the resident solver over the same operator: six loads in one captured group (three 2-load slices on three
     # streams), cold starts as the generic driver's
     from pgj.hybrid import HybridResident
     hres = HybridResident(op).capture()
