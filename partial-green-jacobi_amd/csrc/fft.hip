@@ -681,21 +681,9 @@ extern "C" int pgj_plan_symbol(const pgj_grid* g, const void* plan, double* out,
   return PGJ_OK;
 }
 
-// Cube grids with compile-time FFT kernels run the apply on SoA copies of r
-// and s (the kernels stream one component plane per block); the copies sit
-// after the preconditioner layout: L x 3N doubles (r, then z) and 3N (s).
-namespace {
-size_t precond_soa_offset(const pgj_grid& g, int L) { return precond_layout(g, L).bytes; }
-size_t precond_soa_bytes(const pgj_grid& g, int L) {
-  if (!static_fft(g)) return 0;
-  const size_t n3 = size_t(3) * g.nx * g.ny * g.nz;
-  return align256(size_t(L) * n3 * 8) + align256(n3 * 8);
-}
-}  // namespace
-
 extern "C" size_t pgj_precond_ws_bytes(const pgj_grid* g, int L) {
   if (!g) return 0;
-  return precond_soa_offset(*g, L) + precond_soa_bytes(*g, L);
+  return precond_layout(*g, L).bytes;
 }
 
 extern "C" int pgj_scale_spectrum(const pgj_grid* g, const void* plan, const double* s, void* s_hat, void* ws,
@@ -705,10 +693,9 @@ extern "C" int pgj_scale_spectrum(const pgj_grid* g, const void* plan, const dou
   FFTArgs A = fft_args(*g, plan);
   PrecondLayout pl = precond_layout(*g, 1);
   if (static_fft(*g)) {
-    // compile-time passes on an SoA copy of s (the first SoA slot of the workspace)
-    double* ss = reinterpret_cast<double*>(static_cast<char*>(ws) + precond_soa_offset(*g, 1));
-    if (int e = pgj_aos_to_soa(g, 1, s, ss, stream)) return e;
-    A.s = ss;
+    // the compile-time passes read s in its (1, 3N) node-interleaved layout
+    A.s = s;
+    A.soa = 0;
     A.spec = static_cast<double2*>(s_hat);
     A.pr = reinterpret_cast<double*>(static_cast<char*>(ws) + pl.pr);
     A.pg = reinterpret_cast<double*>(static_cast<char*>(ws) + pl.pg);
@@ -733,24 +720,21 @@ extern "C" int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, con
   FFTArgs A = fft_args(*g, plan);
   PrecondLayout pl = precond_layout(*g, L);
   if (static_fft(*g)) {
+    // the compile-time passes read r and s and write z in their (L, 3N) node-interleaved layout; r is read by
+    // the forward pass only, so z may alias it
     char* w = static_cast<char*>(ws);
-    double* rs = reinterpret_cast<double*>(w + precond_soa_offset(*g, L));
-    double* ss = reinterpret_cast<double*>(w + precond_soa_offset(*g, L) + align256(size_t(L) * 3 * A.N * 8));
     hipStream_t st = as_stream(stream);
-    if (int e = pgj_aos_to_soa(g, L, r, rs, stream)) return e;
-    if (s)
-      if (int e = pgj_aos_to_soa(g, 1, s, ss, stream)) return e;
-    A.rin = rs;
-    A.s = s ? ss : nullptr;
+    A.rin = r;
+    A.s = s;
     A.shat = static_cast<const double2*>(s_hat);
     A.spec = reinterpret_cast<double2*>(w + pl.spec);
     A.pr = reinterpret_cast<double*>(w + pl.pr);
     A.pg = reinterpret_cast<double*>(w + pl.pg);
-    A.zout = rs;  // r is read only by the forward pass
+    A.zout = z;
+    A.soa = 0;
     if (int e = launch_fxy_static(A, L, kFxyPre, st)) return e;
     if (int e = launch_fz_static(A, L, kFzPre, st)) return e;
-    if (int e = launch_fxyi_static(A, L, st, kFxyiPre)) return e;
-    return pgj_soa_to_aos(g, L, rs, z, stream);
+    return launch_fxyi_static(A, L, st, kFxyiPre);
   }
   const int kc = precond_kc(A.f, g->ny, L);
   if (kc != A.f.kc) {

@@ -151,6 +151,16 @@ void fxy_static(FFTArgsT<R> A) {
   X* __restrict__ rp = reinterpret_cast<X*>(A.r + off);
   const X* __restrict__ qp = reinterpret_cast<const X*>(A.q + off);
   const bool scaled = A.s != nullptr;
+  // the scale and preconditioner passes also read the drop-in layout (A.soa == 0): xyz triples per node, so
+  // component c of the pair's two nodes sits 24 bytes apart (the other components' blocks share the lines in L2)
+  const bool aos = MODE != kFxyPCG && !A.soa;
+  const int64_t aoff = int64_t(z) * C::PLANE * 3 + c;
+  const R* __restrict__ s_aos = aos && scaled ? A.s + int64_t(design_of(l, A.group)) * 3 * NN + aoff : nullptr;
+  const R* __restrict__ r_aos = aos && MODE == kFxyPre ? A.rin + int64_t(l) * 3 * NN + aoff : nullptr;
+  auto pair_aos = [](const R* base, int h) {
+    const R* q = base + 6 * int64_t(h);
+    return c2(q[0], q[3]);
+  };
   double alpha = 0.0;
 #pragma unroll
   for (int i0 = 0; i0 < C::ITER; i0 += C::CH) {
@@ -160,9 +170,9 @@ void fxy_static(FFTArgsT<R> A) {
     for (int i = 0; i < C::CH; ++i) {
       const int h = pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1);
       if (MODE == kFxyScale) {
-        va[i] = sp[h];
+        va[i] = aos ? pair_aos(s_aos, h) : sp[h];
       } else if (MODE == kFxyPre) {
-        va[i] = rinp[h];
+        va[i] = aos ? pair_aos(r_aos, h) : rinp[h];
       } else {
         va[i] = rp[h];
         vb[i] = qp[h];
@@ -189,7 +199,10 @@ void fxy_static(FFTArgsT<R> A) {
     if (i0 == 0) twl.store(tw);
     if (MODE != kFxyScale && scaled) {
 #pragma unroll
-      for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+      for (int i = 0; i < C::CH; ++i) {
+        const int h = pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1);
+        vs[i] = aos ? pair_aos(s_aos, h) : sp[h];
+      }
     }
     const R ra = R(alpha);
 #pragma unroll
@@ -668,6 +681,11 @@ void fxyi_static(FFTArgsT<R> A) {
   X* __restrict__ up = move ? reinterpret_cast<X*>(A.u + off) : nullptr;
   X* __restrict__ zp = reinterpret_cast<X*>(A.zout + off);
   const bool scaled = A.s != nullptr;
+  // the preconditioner pass also reads s and writes z in the drop-in layout (A.soa == 0): xyz triples per node
+  const bool aos = MODE == kFxyiPre && !A.soa;
+  const int64_t aoff = int64_t(z) * C::PLANE * 3 + c;
+  const R* __restrict__ s_aos = aos && scaled ? A.s + int64_t(design_of(l, A.group)) * 3 * NN + aoff : nullptr;
+  R* __restrict__ z_aos = aos ? A.zout + int64_t(l) * 3 * NN + aoff : nullptr;
   const R m = R(mean), tb = R(beta);
   X vs[C::CH], vp[C::CH], vu[C::CH];
   auto load_chunk = [&](int i0) {
@@ -679,7 +697,15 @@ void fxyi_static(FFTArgsT<R> A) {
     }
     if (scaled) {
 #pragma unroll
-      for (int i = 0; i < C::CH; ++i) vs[i] = sp[pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1)];
+      for (int i = 0; i < C::CH; ++i) {
+        const int h = pair_of<N>(i0 + i < C::ITER ? i0 + i : C::ITER - 1);
+        if (aos) {
+          const R* q = s_aos + 6 * int64_t(h);
+          vs[i] = c2(q[0], q[3]);
+        } else {
+          vs[i] = sp[h];
+        }
+      }
     }
     if (update || move) {
 #pragma unroll
@@ -728,7 +754,13 @@ void fxyi_static(FFTArgsT<R> A) {
       const X w = plane[y * C::RS + x / 2];
       X v = scaled ? c2(vs[i].x * w.x - m, vs[i].y * w.y - m) : w;
       if (update) v = c2(fma(tb, vp[i].x, v.x), fma(tb, vp[i].y, v.y));
-      zp[h] = v;
+      if (aos) {
+        R* q = z_aos + 6 * int64_t(h);
+        q[0] = v.x;
+        q[3] = v.y;
+      } else {
+        zp[h] = v;
+      }
       if (move) up[h] = c2(fma(ra, vp[i].x, vu[i].x), fma(ra, vp[i].y, vu[i].y));
     }
   }

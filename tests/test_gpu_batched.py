@@ -2,8 +2,11 @@
 
 Every design gets its own seeded filtered-noise density and alpha; each
 design's C^H and per-load iteration counts must match a solo parallel-mode
-solve of the same design (C^H within 1e-10, iterations within 1), and one
-small design is checked against the oracle. Sizes cover the runtime-shaped
+solve of the same design (C^H within 1e-10, iterations within 1; within 2 for
+a tuned set, whose timed row-march shapes and FFT radix plans sum partials in
+another order than the solo solve's defaults, which moves a residual that
+crosses rtol slowly by up to two iterations), and one small design is checked
+against the oracle. Sizes cover the runtime-shaped
 FFT path (8, 12) and the compile-time one (16, 24).
 """
 import numpy as np
@@ -11,6 +14,7 @@ import pytest
 import torch
 
 import pgj
+from pgj import tune
 from pgj.batched import BatchedDesigns
 from oracle import design, green, hex8, pcg
 
@@ -42,7 +46,13 @@ def test_batched_matches_solo(graphs, tuned):
         for d in range(g.D):
             g.material(d, torch.tensor(field(n, d), device='cuda'))
             g.select(d, ALPHAS[(n + d) % 3])
-    infos, _ = bd.solve(rtol=1e-8)
+    try:
+        infos, _ = bd.solve(rtol=1e-8)
+    finally:
+        if tuned:   # the timed choices are process-wide: the solo solves (and later tests) take the defaults
+            for g in bd.groups.values():
+                tune.reset(g.op.grid, g.L)
+    slack = 2 if tuned else 1
     for n, g in bd.groups.items():
         for d in range(g.D):
             solo = pgj.Resident(n, load_mode='parallel', batch=6).capture()
@@ -52,7 +62,7 @@ def test_batched_matches_solo(graphs, tuned):
             for j in range(6):
                 got = infos[n][6 * d + j]
                 assert got['converged'] and got['relative_residual'] <= 1e-8
-                assert abs(got['iterations'] - sinfo[j]['iterations']) <= 1, (n, d, j, got, sinfo[j])
+                assert abs(got['iterations'] - sinfo[j]['iterations']) <= slack, (n, d, j, got, sinfo[j])
             assert rel(g.effective_tensor(d), solo.effective_tensor().cpu().numpy()) < 1e-10, (n, d)
 
 
