@@ -170,7 +170,8 @@ def test_certified_solve_matches_backend(x0):
     assert info['absolute_residual'] <= info['threshold']
     np.testing.assert_allclose(info['rhs_norm'], oinfo['rhs_norm'], rtol=1e-12)
     true = np.linalg.norm(f - hex8.apply(a, u.cpu().numpy(), C0))
-    np.testing.assert_allclose(true, info['absolute_residual'], rtol=1e-6, atol=1e-20)
+    # two evaluations of f - A u (numpy, the device's fixed-order sums) differ by rounding of order eps |f|
+    np.testing.assert_allclose(true, info['absolute_residual'], rtol=1e-4, atol=1e-14 * info['rhs_norm'])
 
 
 def test_effective_stiffness_batched_matches_energy_tensor():
