@@ -30,30 +30,38 @@ def row_march_candidates(n, es):
             if n % zc == 0 and not (p and w > 4 and es == 8)]
 
 
-def tune_row_march(grid, L, es, launch_hex8, reps=6, candidates=None):
+def _fastest(candidates, select, launch, reps, rounds):
+    """{candidate: best mean ms} over ``rounds`` interleaved rounds (each round times every candidate once), so a
+    transient slowdown of the device lands on one round of every candidate rather than on all of one."""
+    times = {}
+    for _ in range(rounds):
+        for cand in candidates:
+            select(cand)
+            t = time_launch(launch, reps)
+            times[cand] = min(times.get(cand, t), t)
+    return times
+
+
+def tune_row_march(grid, L, es, launch_hex8, reps=6, candidates=None, rounds=3):
     """Fastest row-march shape of ``launch_hex8()`` (a HEX8 stage over L loads); recorded and returned."""
     lib = _lib.lib()
-    best = None
-    for shape in candidates or row_march_candidates(grid.nx, es):
-        _lib.check(lib.pgj_hex8_set_shape(grid, L, es, *shape))
-        t = time_launch(launch_hex8, reps)
-        if best is None or t < best[0]:
-            best = (t, shape)
-    _lib.check(lib.pgj_hex8_set_shape(grid, L, es, *best[1]))
-    return dict(shape=best[1], ms=best[0])
+    times = _fastest(list(candidates or row_march_candidates(grid.nx, es)),
+                     lambda shape: _lib.check(lib.pgj_hex8_set_shape(grid, L, es, *shape)), launch_hex8, reps, rounds)
+    shape = min(times, key=times.get)
+    _lib.check(lib.pgj_hex8_set_shape(grid, L, es, *shape))
+    return dict(shape=shape, ms=times[shape])
 
 
-def tune_fft(grid, es, launch_stage, reps=6):
+def tune_fft(grid, es, launch_stage, reps=6, rounds=3):
     """Radix plan (0: radix 4 first, 1: radix 8 first; for the z pass at 64^3 also 2: the transforms in
     registers) of each FFT pass, by timing ``launch_stage(k)`` for stages 1 (forward xy), 2 (z and Green
-    multiply) and 3 (inverse xy); recorded and returned."""
+    multiply) and 3 (inverse xy) in interleaved rounds; recorded and returned."""
     lib = _lib.lib()
     out = {}
     for pas, stage in enumerate((STAGE_FXY, STAGE_FZ, STAGE_FXYI)):
-        times = {}
-        for plan in ((0, 1, 2) if pas == 1 and grid.nx == 64 else (0, 1)):
-            _lib.check(lib.pgj_fft_set_plan(grid, es, pas, plan))
-            times[plan] = time_launch(lambda: launch_stage(stage), reps)
+        plans = (0, 1, 2) if pas == 1 and grid.nx == 64 else (0, 1)
+        times = _fastest(plans, lambda plan: _lib.check(lib.pgj_fft_set_plan(grid, es, pas, plan)),
+                         lambda: launch_stage(stage), reps, rounds)
         plan = min(times, key=times.get)
         _lib.check(lib.pgj_fft_set_plan(grid, es, pas, plan))
         out[('fxy', 'fz', 'fxyi')[pas]] = dict(plan=plan, ms={str(k): v for k, v in times.items()})
