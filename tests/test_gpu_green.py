@@ -123,9 +123,10 @@ def test_set_alpha_attaches_the_centred_log_diagonal():
     assert np.abs(op._scale.cpu().numpy() - s).max() < 1e-13
 
 
-@pytest.mark.parametrize('n', [16, 32, 48, 96, 128])
+@pytest.mark.parametrize('n', [16, 24, 32, 48, 96, 128])
 def test_precond_static_cubes_match_oracle(n):
-    """Cube grids run the generic preconditioner apply through the compile-time FFT passes (SoA copies)."""
+    """Cube grids run the generic preconditioner apply through the compile-time FFT passes, which read r and s
+    and write z in the (L, 3N) node-interleaved layout."""
     rng = np.random.default_rng(n)
     a = np.exp(rng.uniform(-5, 0, (n, n, n)))
     op = pgj.Operator(torch.tensor(a, device='cuda')).enable_green()
