@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # from the FETCH_SIZE / WRITE_SIZE counter runs of `bench.py --profile-stages` scaled by a 256 MiB copy;
 # profiles/pcg_stages_64cube_3load_slices_traffic.md holds the commands, the calibration and the table.
 # Keyed by (n, loads in the set, loads per launch).
-MEASURED_TRAFFIC = {(64, 6, 3): {'hex8_rowmarch': 47.89e6, 'fxy': 82.73e6, 'fz': 61.41e6, 'fxyi': 103.36e6}}
+MEASURED_TRAFFIC = {(64, 6, 3): {'hex8_rowmarch': 47.87e6, 'fxy': 82.73e6, 'fz': 61.41e6, 'fxyi': 103.24e6}}
 
 
 def parse():
