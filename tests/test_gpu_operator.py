@@ -120,3 +120,26 @@ def test_row_march_variants_match_oracle(grid, voxel, zc, w, plane):
         assert (masked[0] == 0).all() and np.array_equal(masked[1], got[1])
     finally:
         _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 0, 0, 0, 0))
+
+
+@pytest.mark.parametrize('grid,voxel', GRIDS, ids=[str(g) for g, _ in GRIDS])
+@pytest.mark.parametrize('zc,w', [(1, 2), (2, 4), (4, 1)])
+def test_two_row_node_march_matches_oracle(grid, voxel, zc, w):
+    """The node march with two element rows per wave (R = 2, a tuner-settable shape; odd ny and two x per lane,
+    nx > 64, fall back to one row) against the oracle, with the load mask."""
+    rng, a, op, vox = make(grid, voxel, 98)
+    u = rng.normal(size=(3, 3 * a.size))
+    ref = hex8.apply(a, u, C0, vox)
+    lib = _lib.lib()
+    _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 2, w, min(zc, grid[0]), 0))
+    try:
+        shape = (_lib.c_int * 5)()
+        _lib.check(lib.pgj_hex8_get_shape(op.grid, 3, 8, shape))
+        assert shape[0] == (2 if grid[1] % 2 == 0 and grid[2] <= 64 else 1) and shape[3] == 0
+        ut = torch.tensor(u, device='cuda')
+        got = op.apply_batched(ut).cpu().numpy()
+        assert rel(got, ref) < 1e-14
+        masked = op.apply_batched(ut, torch.tensor([True, False, True], device='cuda')).cpu().numpy()
+        assert (masked[1] == 0).all() and np.array_equal(masked[2], got[2])
+    finally:
+        _lib.check(lib.pgj_hex8_set_shape(op.grid, 3, 8, 0, 0, 0, 0))
