@@ -681,10 +681,11 @@ def served_64(args):
     server with every client in one resident set of FP32-inner / FP64-refinement slots, certified in FP64 at
     1e-8. Plain: the two-stream FP64 drop-in serving the same requests one at a time with the same warm starts.
     Each path first runs one cold (zero-start) step per client, untimed; then ``--served-steps`` requests per
-    client all waiting at time 0 are timed (closed loop: throughput). Last, Poisson arrivals at 60 % of the
-    served form's measured rate give its p50/p99 request latency under a sustainable load.
+    client all waiting at time 0 are timed (closed loop: throughput). Last, closed-loop clients with exponential
+    think times, offering 60 % of the served form's measured rate, give its p50/p99 request latency under a
+    sustainable load.
     """
-    from pgj.serve import SequentialServer, Server, latency_stats, Client, poisson_arrivals
+    from pgj.serve import SequentialServer, Server, latency_stats, Client
 
     D, S = args.served_designs, args.served_steps
     specs = [(64, args.emin, args.alpha, args.seed + 100 + d) for d in range(D)]
@@ -720,9 +721,12 @@ def served_64(args):
         for ha, hb in zip(a.history[1:], b.history[1:]):
             gap = max(gap, float(np.abs(ha['CH'] - hb['CH']).max() / np.abs(hb['CH']).max()))
     rel = max(h['max_rel'] for c in served_clients for h in c.history)
-    # latency under a sustainable load: Poisson arrivals at 60 % of the closed-loop served rate, after every
-    # client's cold first step (untimed, as in the throughput runs), so the window sees steady-state requests
-    rate = 0.6 * served_rate
+    # latency under a sustainable load, after every client's cold first step (untimed, as in the throughput runs):
+    # closed-loop clients, each submitting its next design step an exponential think time after its previous one
+    # is answered (a design loop needs each step's result), with the mean think time sized so that the offered
+    # rate is 60 % of the closed-loop served rate: D / (D / served_rate + think) = 0.6 served_rate
+    load = 0.6
+    think_mean = D / served_rate * (1.0 / load - 1.0)
     lat_clients = clients(10 ** 6)
     window = 6.0
     del server
@@ -732,8 +736,10 @@ def served_64(args):
     cold_done = lat_server.run(lat_clients, warm)
     cold_s = lat_server.elapsed
     cold = latency_stats(cold_done)
-    arrivals = poisson_arrivals(rate, window, [c.steps - c.done for c in lat_clients], args.seed)
-    lat_done = lat_server.run(lat_clients, arrivals, horizon=window, deadline=3 * window)
+    rng = np.random.default_rng(args.seed)
+    first = sorted((float(rng.exponential(think_mean)), i) for i in range(D))
+    lat_done = lat_server.run(lat_clients, first, horizon=window, deadline=3 * window,
+                              think=lambda c: rng.exponential(think_mean))
     stats = latency_stats(lat_done)
     return {
         'workload': f'{D} clients at 64^3, Emin {args.emin:g}, alpha {args.alpha:g}, radius-4 filtered noise '
@@ -749,10 +755,13 @@ def served_64(args):
         'speedup_vs_plain': served_rate / plain_rate,
         'tuned': {'served': server_tuned, 'plain': base.tuned},
         'ch_gap_vs_plain_max_rel': gap,
-        'serve_latency': {'arrival_rate_per_s': rate, 'window_s': window, 'completed': stats['count'],
-                          'note': 'warm-started design steps in steady state: every client has taken its cold '
-                                  '(zero-start) first step before the window; each client has one request in '
-                                  'flight at a time',
+        'serve_latency': {'offered_rate_per_s': load * served_rate, 'mean_think_ms': 1e3 * think_mean,
+                          'window_s': window, 'completed': stats['count'],
+                          'completed_per_s': stats['count'] / lat_server.elapsed,
+                          'note': f'closed-loop clients: each submits its next warm-started design step an '
+                                  f'exponential think time after its previous one is answered, the mean sized for '
+                                  f'{load:.0%} of the closed-loop served rate; every client has taken its cold '
+                                  f'(zero-start) first step before the window',
                           'p50_ms': 1e3 * stats['p50'], 'p99_ms': 1e3 * stats['p99'],
                           'unserved': lat_server.unserved,
                           'cold_first_steps': {'requests': cold['count'], 'all_arriving_at_s': 0.0,

@@ -296,13 +296,17 @@ class Server:
         return True
 
     # -- the loop -----------------------------------------------------------------------
-    def run(self, clients, arrivals, horizon=None, deadline=None, clock=None):
+    def run(self, clients, arrivals, horizon=None, deadline=None, clock=None, think=None):
         """Serve ``arrivals`` (sorted (time, client) pairs, time in seconds from start) until all are done.
 
         ``horizon`` drops arrivals after that many seconds; after ``deadline`` seconds no further request is
         started (the ones in flight finish; the rest count in ``self.unserved``). Returns the completed requests.
-        ``clock`` (a callable giving seconds) replaces the wall clock.
+        ``clock`` (a callable giving seconds) replaces the wall clock. ``think`` (a callable client -> seconds)
+        makes the clients closed-loop: once a client's request is answered, it submits its next step that many
+        seconds later (while it has steps left and within the horizon), as a design loop that needs each step's
+        result before the next.
         """
+        index = {id(c): i for i, c in enumerate(clients)}
         for c in clients:                       # OC graphs of every (grid, Emin) before the clock starts
             self._worker(c.n, c.emin)
         torch.cuda.synchronize()
@@ -356,8 +360,14 @@ class Server:
             t3 = time.perf_counter()
             for n, slot in finished:
                 req = self.solving[n][slot]
-                if self._complete(n, slot, now()) and not req.failed:
-                    done.append(req)
+                if self._complete(n, slot, now()):
+                    if not req.failed:
+                        done.append(req)
+                    c = req.client
+                    if think is not None and c.done + c.failed < c.steps:
+                        at = now() + float(think(c))
+                        if horizon is None or at <= horizon:
+                            heapq.heappush(heap, (at, index[id(c)]))
             t4 = time.perf_counter()
             self.phases['advance'] += t2 - t1
             self.phases['status'] += t3 - t2

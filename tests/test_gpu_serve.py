@@ -86,3 +86,18 @@ def test_sequential_baseline_serves_in_order():
     assert len(done) == 3 and base.unserved == 0
     assert [len(x.history) for x in c] == [2, 1]
     assert all(h['max_rel'] <= 1e-8 for x in c for h in x.history)
+
+
+def test_closed_loop_clients_submit_after_each_answer():
+    """think=...: each client's next request arrives a think time after its previous one was answered, so no
+    request waits behind its own client's step; every step is certified."""
+    c = clients()[:2]
+    server = Server(SIZES, BUDGET, max_slots=2)
+    done = server.run(c, [(0.0, 0), (0.0, 1)], think=lambda client: 0.002)
+    assert len(done) == 2 * STEPS and server.unserved == 0
+    for client in c:
+        reqs = sorted((r for r in done if r.client is client), key=lambda r: r.arrival)
+        assert len(reqs) == STEPS
+        for prev, nxt in zip(reqs, reqs[1:]):
+            assert nxt.arrival >= prev.end + 0.002 - 1e-9
+    assert all(h['max_rel'] <= 1e-8 for x in c for h in x.history)
