@@ -597,9 +597,12 @@ def configs_form(args):
     rows['16^3 serial alpha 0.5']['ch_rel_err_vs_fp64_oracle'] = float(np.abs(H16 - H_ref).max() /
                                                                       np.abs(H_ref).max())
     a64 = simp(filtered_noise(64, args.seed, 4), 1e-6)
+    H64 = None
     for alpha in (0.0, 0.5, 1.0):
         for mode in ('serial', 'parallel'):
-            rows[f'64^3 {mode} alpha {alpha:g}'], _ = timed(64, a64, alpha, mode, 2)
+            rows[f'64^3 {mode} alpha {alpha:g}'], H = timed(64, a64, alpha, mode, 2)
+            if alpha == 0.5 and mode == 'parallel':
+                H64 = H
     # the generic path: Operator + set_alpha + certified_solve (Backend.solve), one load at a time
     op = pgj.build(torch.tensor(a64, device='cuda'))
     pgj.set_alpha(op, 0.5)
@@ -612,6 +615,25 @@ def configs_form(args):
     rows['64^3 generic certified_solve alpha 0.5'] = {
         'solves_per_s': 1.0 / (time.perf_counter() - t0), 'iterations': [g['iterations'] for g in gen],
         'max_relative_residual': max(g['relative_residual'] for g in gen)}
+    # the iterative-refinement drivers over an FP32 inner operator (reference defaults: inner tol 0.1, at most 80
+    # inner iterations per outer step), outer tol 1e-8 on the FP64 residual; C^H by mean stress
+    from pgj import ir
+    op32 = pgj.build(torch.tensor(a64, device='cuda'), dtype=torch.float32)
+    pgj.set_alpha(op32, 0.5)
+    ir.solve_ir_batched(op, op32, rhs[0], outer_tol=1e-8)    # untimed: first launches of the FP32 kernels
+    for name, fn in (('compute_effective_stiffness_ir_batched', ir.compute_effective_stiffness_ir_batched),
+                     ('compute_effective_stiffness_ir_adaptive_batched',
+                      ir.compute_effective_stiffness_ir_adaptive_batched)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        C, info = fn(op, op32, outer_tol=1e-8)
+        torch.cuda.synchronize()
+        rows[f'64^3 {name} alpha 0.5'] = {
+            'solves_per_s': 1.0 / (time.perf_counter() - t0),
+            'outer_iterations': [e['outer_iters'] for e in info[0]],
+            'inner_iterations': [e['inner_iters'] for e in info[0]],
+            'max_relative_residual': max(e['rel_res'] for e in info[0]),
+            'ch_rel_vs_resident': float(np.abs(C[0] - H64).max() / np.abs(H64).max())}
     out = {'metric': 'cold certified 6-load solves/s per configuration (rtol 1e-8)', 'unit': 'certified 6-load solves/s',
            'n_gpus': 1, 'dtype': 'f64', 'data': 'synthetic (seeded filtered-noise densities)',
            'higher_is_better': True, 'configs': rows}
