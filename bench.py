@@ -46,6 +46,9 @@ def parse():
     p.add_argument('--move', type=float, default=0.2)
     p.add_argument('--rmin', type=float, default=3.0)
     p.add_argument('--probe-iters', type=int, default=10)
+    p.add_argument('--fi-precision', choices=('mixed', 'fp64'), default='mixed',
+                   help="hybrid form: enable_fi_reference precision ('mixed', the reference's default, applies the "
+                        'Green operator in FP32 on compile-time cubes)')
     p.add_argument('--streams', type=int, default=2,
                    help='streams the parallel load group is split over in the solve (slices of batch/streams loads)')
     p.add_argument('--cpu-seconds', type=float, default=12.0)
@@ -430,7 +433,7 @@ def hybrid_form(args):
     phase, materials, orientation = composite(n, args.seed, args.radius)
     pool, index, meta = build_hybrid_material(phase, materials, orientation=orientation)
     grid = (n, n, n)
-    op = enable_fi_reference(HybridFIOperator(pool, index, grid, (1.0 / n,) * 3))
+    op = enable_fi_reference(HybridFIOperator(pool, index, grid, (1.0 / n,) * 3), precision=args.fi_precision)
     N = n ** 3
     if args.profile_stages:
         # counter-run workload: one 256 MiB device copy (calibration), then the one-load tiled apply
@@ -513,8 +516,11 @@ def hybrid_form(args):
         'dtype': 'f64', 'data': 'synthetic (three phases by quartiles of seeded filtered noise)',
         'config': {'workload': f'{n}^3 three-phase composite: isotropic matrix (E 1, nu 0.3) 50%, local-frame '
                                'orthotropic fibres (frame turning with x, one pool row per voxel) 25%, isotropic '
-                               'inclusions (E 10, nu 0.2) 25%; zeman reference-medium FI Green preconditioner; '
-                               'solve_pcg_batched eager, tol 1e-8', 'n': n, 'pool_rows': meta['pool_rows']},
+                               'inclusions (E 10, nu 0.2) 25%; zeman reference-medium FI Green preconditioner '
+                               f'(enable_fi_reference precision {args.fi_precision!r}: '
+                               f'{"FP32 apply" if op._plan32 is not None else "FP64 apply"}); '
+                               'solve_pcg_batched eager, tol 1e-8', 'n': n, 'pool_rows': meta['pool_rows'],
+                   'green_apply': 'fp32' if op._plan32 is not None else 'fp64'},
         'latency_ms': {'p50': float(np.percentile(lat, 50) * 1e3), 'p99': float(np.percentile(lat, 99) * 1e3)},
         'iterations': iters, 'max_rel_res': res, 'CH_diag': np.diag(C[0]).tolist(),
         'apply_us': {'tile_L1': t1, 'tile_L6': t6, 'gather_L1': g1},

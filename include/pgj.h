@@ -186,6 +186,14 @@ int pgj_scale_spectrum(const pgj_grid* g, const void* plan, const double* s, voi
  *  solver/experiments/adaptive_green_scaling/study.py:91-101). */
 int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, const double* r, const double* s,
                       const void* s_hat, double* z, void* ws, void* stream);
+/* The same apply in FP32 arithmetic on FP64 (L, 3N) vectors: r is rounded to
+ * FP32 as it is read, z is the FP32 result widened to FP64. plan32 from
+ * pgj_plan32_init, s and s_hat FP32 / complex64 (nullable together), the
+ * pgj_precond_ws_bytes workspace; cubic n in 16, 24, 32, 48, 64, 96, 128.
+ * Replaces the 'mixed' (default) Green apply of enable_fi_reference
+ * (solver/src/reference_medium/formula_adapter.py:40, :59-66). */
+int pgj_precond_apply_mixed(const pgj_grid* g, int L, const void* plan32, const double* r, const float* s,
+                            const void* s_hat, double* z, void* ws, void* stream);
 
 /* ------------------------------------------------------ resident PCG (FP64) */
 

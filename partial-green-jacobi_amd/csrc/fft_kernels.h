@@ -45,6 +45,8 @@ struct FFTArgsT {
   int zwide;             // z pass variant of a cube (zpass_variant): 0 default kx chunks, 1 wide chunks,
                          // kZReg register-resident; the same for every launch that writes or reads one
                          // workspace's z partials
+  const double* rin64;   // FP32 preconditioner apply on AoS vectors: FP64 input residual, rounded to T on load
+  double* zout64;        // ... and FP64 output z (exact widening of the T result); both null otherwise
 };
 using FFTArgs = FFTArgsT<double>;
 

@@ -156,7 +156,10 @@ void fxy_static(FFTArgsT<R> A) {
   const bool aos = MODE != kFxyPCG && !A.soa;
   const int64_t aoff = int64_t(z) * C::PLANE * 3 + c;
   const R* __restrict__ s_aos = aos && scaled ? A.s + int64_t(design_of(l, A.group)) * 3 * NN + aoff : nullptr;
-  const R* __restrict__ r_aos = aos && MODE == kFxyPre ? A.rin + int64_t(l) * 3 * NN + aoff : nullptr;
+  const R* __restrict__ r_aos = aos && MODE == kFxyPre && A.rin ? A.rin + int64_t(l) * 3 * NN + aoff : nullptr;
+  // FP32 apply of FP64 vectors: the residual is read in FP64 and rounded to R
+  const bool wide_in = sizeof(R) == 4 && aos && MODE == kFxyPre && A.rin64 != nullptr;
+  const double* __restrict__ r64 = wide_in ? A.rin64 + int64_t(l) * 3 * NN + aoff : nullptr;
   auto pair_aos = [](const R* base, int h) {
     const R* q = base + 6 * int64_t(h);
     return c2(q[0], q[3]);
@@ -172,7 +175,12 @@ void fxy_static(FFTArgsT<R> A) {
       if (MODE == kFxyScale) {
         va[i] = aos ? pair_aos(s_aos, h) : sp[h];
       } else if (MODE == kFxyPre) {
-        va[i] = aos ? pair_aos(r_aos, h) : rinp[h];
+        if (wide_in) {
+          const double* q = r64 + 6 * int64_t(h);
+          va[i] = c2(R(q[0]), R(q[3]));
+        } else {
+          va[i] = aos ? pair_aos(r_aos, h) : rinp[h];
+        }
       } else {
         va[i] = rp[h];
         vb[i] = qp[h];
@@ -685,7 +693,9 @@ void fxyi_static(FFTArgsT<R> A) {
   const bool aos = MODE == kFxyiPre && !A.soa;
   const int64_t aoff = int64_t(z) * C::PLANE * 3 + c;
   const R* __restrict__ s_aos = aos && scaled ? A.s + int64_t(design_of(l, A.group)) * 3 * NN + aoff : nullptr;
-  R* __restrict__ z_aos = aos ? A.zout + int64_t(l) * 3 * NN + aoff : nullptr;
+  R* __restrict__ z_aos = aos && A.zout ? A.zout + int64_t(l) * 3 * NN + aoff : nullptr;
+  // FP32 apply of FP64 vectors: z is written in FP64
+  double* __restrict__ z64 = sizeof(R) == 4 && aos && A.zout64 ? A.zout64 + int64_t(l) * 3 * NN + aoff : nullptr;
   const R m = R(mean), tb = R(beta);
   X vs[C::CH], vp[C::CH], vu[C::CH];
   auto load_chunk = [&](int i0) {
@@ -754,7 +764,11 @@ void fxyi_static(FFTArgsT<R> A) {
       const X w = plane[y * C::RS + x / 2];
       X v = scaled ? c2(vs[i].x * w.x - m, vs[i].y * w.y - m) : w;
       if (update) v = c2(fma(tb, vp[i].x, v.x), fma(tb, vp[i].y, v.y));
-      if (aos) {
+      if (z64) {
+        double* q = z64 + 6 * int64_t(h);
+        q[0] = double(v.x);
+        q[3] = double(v.y);
+      } else if (aos) {
         R* q = z_aos + 6 * int64_t(h);
         q[0] = v.x;
         q[3] = v.y;

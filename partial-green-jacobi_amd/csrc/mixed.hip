@@ -112,6 +112,31 @@ extern "C" int pgj_plan32_init(const pgj_grid* g, const void* plan, void* plan32
   return PGJ_OK;
 }
 
+extern "C" int pgj_precond_apply_mixed(const pgj_grid* g, int L, const void* plan32, const double* r, const float* s,
+                                       const void* s_hat, double* z, void* ws, void* stream) {
+  PGJ_REQUIRE(g && static_cube(*g), "the FP32 preconditioner apply needs a cubic n in 16, 24, 32, 48, 64, 96, 128");
+  PGJ_REQUIRE(L >= 1 && L <= 65535 && plan32 && r && z && ws, "invalid FP32 preconditioner arguments");
+  PGJ_REQUIRE((s == nullptr) == (s_hat == nullptr), "s and s_hat must be given together");
+  // the compile-time FP32 passes on the (L, 3N) node-interleaved vectors, r rounded to FP32 as it is read and z
+  // written in FP64; the FP64 preconditioner workspace holds the complex64 spectrum with room to spare and the
+  // same FP64 partial rows
+  FFTArgsT<float> A = fft_args32(*g, plan32);
+  const PrecondLayout pl = precond_layout(*g, L);
+  char* w = static_cast<char*>(ws);
+  A.rin64 = r;
+  A.s = s;
+  A.shat = static_cast<const float2*>(s_hat);
+  A.spec = reinterpret_cast<float2*>(w + pl.spec);
+  A.pr = reinterpret_cast<double*>(w + pl.pr);
+  A.pg = reinterpret_cast<double*>(w + pl.pg);
+  A.zout64 = z;
+  A.soa = 0;
+  hipStream_t st = as_stream(stream);
+  if (int e = launch_fxy_static(A, L, kFxyPre, st)) return e;
+  if (int e = launch_fz_static(A, L, kFzPre, st)) return e;
+  return launch_fxyi_static(A, L, st, kFxyiPre);
+}
+
 extern "C" size_t pgj_inner_ws_bytes(const pgj_grid* g, int L) { return resident_layout(*g, L, 8).bytes; }
 
 extern "C" int pgj_inner_begin(const pgj_grid* g, int L, int l0, int nl, const double* r64, const void* state64,

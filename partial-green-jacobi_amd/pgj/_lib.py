@@ -59,6 +59,8 @@ SIGNATURES = {
     'pgj_fi_coeff': (c_int, [GP, c_void_p, c_void_p, c_void_p]),
     'pgj_precond_ws_bytes': (c_size_t, [GP, c_int]),
     'pgj_scale_spectrum': (c_int, [GP, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'pgj_precond_apply_mixed': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
     'pgj_precond_apply': (c_int, [GP, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p]),
     'pgj_resident_state_bytes': (c_size_t, [c_int]),

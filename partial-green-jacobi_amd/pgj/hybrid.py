@@ -172,10 +172,16 @@ class HybridFIOperator:
         return out[None]
 
 
+# cubes with compile-time FFT passes, the grids of the FP32 apply
+FP32_CUBES = (16, 24, 32, 48, 64, 96, 128)
+
+
 def enable_fi_reference(op, precision='mixed', layout='upper6'):
     """Attach ``apply_precond`` = the FI Green operator of the isotropic reference medium
-    (``formula_adapter.py:40-68``). The spectral apply is FP64 for both ``precision`` values
-    (libpgj's LDS FFT); ``op._formula_precision`` records the request."""
+    (``formula_adapter.py:40-68``). ``precision='mixed'`` (the default) applies it in FP32 as the reference
+    does (``formula_adapter.py:59-66``: the residual rounded to FP32, FP32 transforms and symbol, the result
+    returned in the residual's dtype) on the cubes in ``FP32_CUBES``, and in FP64 on other grids;
+    ``'fp64'`` applies it in FP64 everywhere."""
     if precision not in ('mixed', 'fp64'):
         raise ValueError(precision)
     reference = op._compute_reference_medium('zeman', verbose=False)[0]
@@ -192,6 +198,10 @@ def enable_fi_reference(op, precision='mixed', layout='upper6'):
     op._plan = torch.empty(lib.pgj_plan_bytes(op.grid), dtype=torch.uint8, device=op.device)
     _lib.call('pgj_plan_init', op.grid, _lib.ptr(op._plan), _lib.stream())
     op._pre_ws = torch.empty(lib.pgj_precond_ws_bytes(op.grid, 1), dtype=torch.uint8, device=op.device)
+    op._plan32 = None
+    if precision == 'mixed' and shape[0] == shape[1] == shape[2] and shape[0] in FP32_CUBES:
+        op._plan32 = torch.empty(lib.pgj_plan32_bytes(op.grid), dtype=torch.uint8, device=op.device)
+        _lib.call('pgj_plan32_init', op.grid, _lib.ptr(op._plan), _lib.ptr(op._plan32), _lib.stream())
     op._formula_reference_C0 = reference.tolist()
     op._fft_enabled = True
     op._fft_spectrum_mode = 'real_half'
@@ -203,6 +213,11 @@ def enable_fi_reference(op, precision='mixed', layout='upper6'):
 
     def apply(self, residual):
         r = residual.reshape(1, -1).contiguous()
+        if self._plan32 is not None:
+            z = torch.empty_like(r)
+            _lib.call('pgj_precond_apply_mixed', self.grid, 1, _lib.ptr(self._plan32), _lib.ptr(r), None, None,
+                      _lib.ptr(z), _lib.ptr(self._pre_ws), _lib.stream())
+            return z.view_as(residual)
         z = torch.empty_like(r)
         _lib.call('pgj_precond_apply', self.grid, 1, _lib.ptr(self._plan), _lib.ptr(r), None, None, _lib.ptr(z),
                   _lib.ptr(self._pre_ws), _lib.stream())

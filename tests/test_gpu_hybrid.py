@@ -170,3 +170,54 @@ def test_argument_errors():
         op.apply(torch.zeros(2, 3 * 64, dtype=torch.float64, device='cuda'))
     with pytest.raises(ValueError, match='Six'):
         op.compute_rhs(np.zeros(5))
+
+
+@pytest.mark.parametrize('n', [16, 64])
+def test_fi_reference_mixed_precision_applies_in_fp32(n):
+    """precision='mixed' (the reference's default) applies the FI Green operator in FP32 on compile-time cubes:
+    within FP32 rounding of the oracle's FP64 Green (and not within FP64 rounding), returned in FP64; 'fp64'
+    matches the oracle to 1e-13. Tolerance of the FP32 apply: 2e-5 of max|z| (FP32 transforms of 2n-point
+    rows)."""
+    from oracle import green
+    grid = (n, n, n)
+    rng = np.random.default_rng(17)
+    phase = rng.choice([0, 1, 2], size=grid).astype(np.int32)
+    materials = {0: dict(kind='isotropic', E=1.0, nu=0.3), 1: dict(kind='isotropic', E=10.0, nu=0.2),
+                 2: dict(kind='anisotropic', C=spd(rng, 0.5))}
+    pool, index, _ = mat.build_hybrid_material(phase, materials, device='cuda')
+    vox = hex8.voxel_of(grid)
+    mixed = enable_fi_reference(HybridFIOperator(pool, index, grid, vox))
+    full = enable_fi_reference(HybridFIOperator(pool, index, grid, vox), precision='fp64')
+    assert mixed._plan32 is not None and full._plan32 is None
+    ref_C0 = np.asarray(mixed._formula_reference_C0)
+    r = np.random.default_rng(n).standard_normal((1, 3 * n ** 3))
+    rt = torch.tensor(r, device='cuda')
+    zm, zf = mixed.apply_precond(rt), full.apply_precond(rt)
+    assert zm.dtype == torch.float64 and zm.shape == rt.shape
+    want = green.precond(r, green.symbol(grid, vox, ref_C0[0, 1], ref_C0[3, 3]), grid, None)
+    assert rel(zf.cpu().numpy(), want) < 1e-13
+    err = rel(zm.cpu().numpy(), want)
+    assert 1e-10 < err < 2e-5, err
+    # the FP32 result depends only on the FP32-rounded residual
+    again = mixed.apply_precond(rt.float().double())
+    assert torch.equal(again, zm)
+
+
+def test_fi_reference_mixed_solve_and_other_grids():
+    """The generic PCG with the FP32 Green apply still certifies in FP64 (the residual is FP64); a grid outside
+    the compile-time cubes takes the FP64 apply for precision='mixed'."""
+    grid = (16, 16, 16)
+    _, pool, index, vox = multiphase(grid, 19)
+    op = enable_fi_reference(HybridFIOperator(pool, index, grid, vox))
+    rhs = op.compute_rhs(np.eye(6)[3])
+    u, iters, res = pgj.solve_pcg_batched(op, rhs, tol=1e-10, max_iter=2000)
+    f = rhs.cpu().numpy()
+    r = f - hybrid.apply(pool.cpu().numpy(), index[0].cpu().numpy(), u.cpu().numpy(), grid, vox)
+    assert np.linalg.norm(r) / np.linalg.norm(f) < 1e-9 and float(res.max()) < 1e-10
+    grid = (6, 8, 10)
+    _, pool, index, vox = multiphase(grid, 23)
+    mixed = enable_fi_reference(HybridFIOperator(pool, index, grid, vox))
+    full = enable_fi_reference(HybridFIOperator(pool, index, grid, vox), precision='fp64')
+    assert mixed._plan32 is None
+    rt = torch.tensor(np.random.default_rng(3).standard_normal((1, 3 * 480)), device='cuda')
+    assert torch.equal(mixed.apply_precond(rt), full.apply_precond(rt))
