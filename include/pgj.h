@@ -188,7 +188,8 @@ int pgj_precond_apply(const pgj_grid* g, int L, const void* plan, const double* 
                       const void* s_hat, double* z, void* ws, void* stream);
 /* The same apply in FP32 arithmetic on FP64 (L, 3N) vectors: r is rounded to
  * FP32 as it is read, z is the FP32 result widened to FP64. plan32 from
- * pgj_plan32_init, s and s_hat FP32 / complex64 (nullable together), the
+ * pgj_plan32_init, s and s_hat FP32 / complex64 (nullable together; s_hat is
+ * pgj_scale_spectrum's output rounded by pgj_to_f32), the
  * pgj_precond_ws_bytes workspace; cubic n in 16, 24, 32, 48, 64, 96, 128.
  * Replaces the 'mixed' (default) Green apply of enable_fi_reference
  * (solver/src/reference_medium/formula_adapter.py:40, :59-66). */

@@ -138,3 +138,35 @@ def test_precond_static_cubes_match_oracle(n):
     assert rel(z, ref) < 1e-12
     one = op.apply_precond(torch.tensor(r[1:2], device='cuda')).cpu().numpy()
     assert np.array_equal(one[0], z[1])   # the compile-time passes have one block shape for any load count
+
+
+@pytest.mark.parametrize('n', [32, 64])
+def test_mixed_apply_scaled_matches_oracle(n):
+    """pgj_precond_apply_mixed (FP32 arithmetic on FP64 vectors) with the FP32-rounded scale and its spectrum,
+    two loads: within FP32 rounding of the oracle's FP64 M_alpha (2-norm relative 1e-5, and above 1e-10, so the
+    FP32 passes ran)."""
+    from pgj import _lib
+    rng = np.random.default_rng(n + 5)
+    a = np.exp(rng.uniform(-5, 0, (n, n, n)))
+    op = pgj.Operator(torch.tensor(a, device='cuda')).enable_green()
+    pgj.set_alpha(op, 0.5)
+    lib = _lib.lib()
+    plan32 = torch.empty(lib.pgj_plan32_bytes(op.grid), dtype=torch.uint8, device='cuda')
+    _lib.call('pgj_plan32_init', op.grid, _lib.ptr(op._plan), _lib.ptr(plan32), _lib.stream())
+    s32 = op._scale.float().contiguous()
+    shat32 = torch.empty(2 * op._scale_hat.numel(), dtype=torch.float32, device='cuda')
+    _lib.call('pgj_to_f32', 2 * op._scale_hat.numel(), _lib.ptr(op._scale_hat), _lib.ptr(shat32), _lib.stream())
+    r = rng.normal(size=(2, 3 * n ** 3))
+    rt = torch.tensor(r, device='cuda')
+    z = torch.empty_like(rt)
+    ws = torch.empty(lib.pgj_precond_ws_bytes(op.grid, 2), dtype=torch.uint8, device='cuda')
+    _lib.call('pgj_precond_apply_mixed', op.grid, 2, _lib.ptr(plan32), _lib.ptr(rt), _lib.ptr(s32), _lib.ptr(shat32),
+              _lib.ptr(z), _lib.ptr(ws), _lib.stream())
+    grid = (n, n, n)
+    ref = green.precond(r, green.symbol(grid, hex8.voxel_of(grid), LAM, MU), grid, green.scale(a, 0.5, C0))
+    err = rel(z.cpu().numpy(), ref)
+    assert 1e-10 < err < 1e-5, err
+    with pytest.raises(ValueError):
+        odd = pgj.Operator(torch.ones((6, 8, 10), dtype=torch.float64, device='cuda')).enable_green()
+        _lib.call('pgj_precond_apply_mixed', odd.grid, 1, _lib.ptr(plan32), _lib.ptr(rt), None, None, _lib.ptr(z),
+                  _lib.ptr(ws), _lib.stream())
