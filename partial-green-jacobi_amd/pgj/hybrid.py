@@ -233,8 +233,9 @@ class HybridResident:
 
     The recurrence, masks, iteration cap, restarts and true-residual certification are those of
     ``pgj.Resident`` (``exp/asg_gpu_resident_20261103/resident.py:39-58``, ``:186-209``). The preconditioner is
-    the operator's unscaled FI Green operator, as ``apply_precond``. The loads of a group advance together in
-    hipGraph-captured chunks of ``chunk`` iterations:
+    the operator's unscaled FI Green operator, as ``apply_precond``, applied in FP64 by the fused stages whatever
+    ``enable_fi_reference``'s ``precision`` (its FP32 apply serves the generic path). The loads of a group
+    advance together in hipGraph-captured chunks of ``chunk`` iterations:
     - ``q = K p`` is the tiled C21 apply with fused p.q partials;
     - the three FFT passes are those of the scalar solver;
     - certification uses the gather apply.
