@@ -140,7 +140,7 @@ def test_precond_static_cubes_match_oracle(n):
     assert np.array_equal(one[0], z[1])   # the compile-time passes have one block shape for any load count
 
 
-@pytest.mark.parametrize('n', [32, 64])
+@pytest.mark.parametrize('n', [32, 48, 64, 96])
 def test_mixed_apply_scaled_matches_oracle(n):
     """pgj_precond_apply_mixed (FP32 arithmetic on FP64 vectors) with the FP32-rounded scale and its spectrum,
     two loads: within FP32 rounding of the oracle's FP64 M_alpha (2-norm relative 1e-5, and above 1e-10, so the

@@ -172,7 +172,7 @@ def test_argument_errors():
         op.compute_rhs(np.zeros(5))
 
 
-@pytest.mark.parametrize('n', [16, 64])
+@pytest.mark.parametrize('n', [16, 24, 64, 128])
 def test_fi_reference_mixed_precision_applies_in_fp32(n):
     """precision='mixed' (the reference's default) applies the FI Green operator in FP32 on compile-time cubes:
     within FP32 rounding of the oracle's FP64 Green (and not within FP64 rounding), returned in FP64; 'fp64'
