@@ -725,7 +725,7 @@ def served_64(args):
     timed = [(0.0, i) for i in range(D) for _ in range(S)]
     # served: one size class holding all D designs
     served_clients = clients(1 + S)
-    server = Server((64,), 1e12, max_slots=D, precision='mixed', volume=args.volume, move=args.move,
+    server = Server((64,), 1e12, max_slots=D, precision=args.serve_precision, volume=args.volume, move=args.move,
                     rmin=args.rmin, tune=bool(args.tune))
     server.run(served_clients, warm)
     torch.cuda.synchronize()
@@ -759,7 +759,7 @@ def served_64(args):
     window = 6.0
     del server
     torch.cuda.empty_cache()
-    lat_server = Server((64,), 1e12, max_slots=D, precision='mixed', volume=args.volume, move=args.move,
+    lat_server = Server((64,), 1e12, max_slots=D, precision=args.serve_precision, volume=args.volume, move=args.move,
                         rmin=args.rmin)             # the tuned shapes and plans of the first server still hold
     cold_done = lat_server.run(lat_clients, warm)
     cold_s = lat_server.elapsed
