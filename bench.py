@@ -49,6 +49,9 @@ def parse():
     p.add_argument('--fi-precision', choices=('mixed', 'fp64'), default='mixed',
                    help="hybrid form: enable_fi_reference precision ('mixed', the reference's default, applies the "
                         'Green operator in FP32 on compile-time cubes)')
+    p.add_argument('--chunk', type=int, default=4,
+                   help='PCG iterations per captured graph of the plain solve (the reference uses 4; loads stop on '
+                        'their own masks, so results do not depend on it)')
     p.add_argument('--streams', type=int, default=2,
                    help='streams the parallel load group is split over in the solve (slices of batch/streams loads)')
     p.add_argument('--cpu-seconds', type=float, default=12.0)
@@ -258,7 +261,8 @@ def profile_stages(args):
     if args.profile_precision == 'fp32':
         return profile_stages_fp32(args)
     n = args.n
-    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=args.batch, streams=args.streams).capture()
+    res = pgj.Resident(n, loads=np.eye(6), load_mode='parallel', batch=args.batch, streams=args.streams,
+                       chunk=args.chunk).capture()
     res.material(torch.tensor(simp(filtered_noise(n, args.seed, args.radius), args.emin), device='cuda'))
     res.select(args.alpha)
     parts = res._halves.parts
